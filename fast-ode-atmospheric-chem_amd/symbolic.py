@@ -1,0 +1,438 @@
+"""Host symbolic analysis of the stage matrix alpha*M - J for the device LU.
+
+The reference factors alpha*M - J with SuperLU (``permc_spec="MMD_AT_PLUS_A"``,
+partial pivoting) on a block-diagonal matrix of the group's cells
+(fastode/linalg.py:70-81), or with LAPACK getrf per cell (linalg.py:15-21).
+The device factors each cell's matrix in place *without pivoting*, on a fill
+pattern fixed here once per mechanism:
+
+1. order: minimum degree on the pattern of A + A^T (ties to the lowest index),
+   the same family of ordering SuperLU is asked for;
+2. fill: symbolic elimination without pivoting gives the L+U pattern;
+3. split: pivots [0, c) are *sparse* and [c, n) form the *core*, a trailing
+   block that fill has made nearly dense.  The core is factored as a dense
+   block held in registers, distributed 2-D cyclically over a G x G thread
+   grid with TILE x TILE entries per thread (m_pad = G * TILE >= n - c,
+   padded with identity rows);
+4. sparse entries (min(i, j) < c) live in LDS and are computed by
+   dot-product tasks  A_ij = (A_ij - sum_{p < min(i,j)} L_ip U_pj) [/ U_jj],
+   grouped in stages: a task runs once every task it reads has run;
+5. the core's Schur complement update from the sparse pivots,
+   A_ij -= sum_{p < c} L_ip U_pj, is a per-(slot, thread) list of pairs;
+6. the solves use level-scheduled row tasks for the sparse rows and block
+   substitution with inverted diagonal blocks for the core.
+
+Everything the kernels read is emitted here as flat integer arrays (see
+``LUPlan.arrays``); the C ABI copies them to the device once.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+G = 16                 # core thread grid is G x G (256 threads)
+TILE_MAX = 10          # largest supported core: G * TILE_MAX = 160
+DENSE_RHO = 0.85       # core = longest trailing block at least this dense
+
+
+def min_degree_order(n, rows, cols):
+    """Minimum degree ordering of the graph of A + A^T (lowest index on ties).
+    Returns perm with perm[new] = old."""
+    Gm = np.zeros((n, n), dtype=bool)
+    Gm[rows, cols] = True
+    Gm |= Gm.T
+    np.fill_diagonal(Gm, False)
+    alive = np.ones(n, dtype=bool)
+    order = []
+    for _ in range(n):
+        deg = np.where(alive, Gm.sum(axis=1), np.iinfo(np.int64).max)
+        v = int(np.argmin(deg))
+        order.append(v)
+        nb = np.nonzero(Gm[v] & alive)[0]
+        Gm[np.ix_(nb, nb)] = True
+        Gm[nb, nb] = False
+        alive[v] = False
+        Gm[v, :] = False
+        Gm[:, v] = False
+    return np.asarray(order, dtype=np.int64)
+
+
+def symbolic_fill(n, rows_new, cols_new):
+    """Filled L+U pattern of no-pivot elimination, as a dense bool matrix."""
+    F = np.zeros((n, n), dtype=bool)
+    F[rows_new, cols_new] = True
+    F[np.arange(n), np.arange(n)] = True
+    for k in range(n):
+        L = np.nonzero(F[k + 1:, k])[0] + k + 1
+        U = np.nonzero(F[k, k + 1:])[0] + k + 1
+        if len(L) and len(U):
+            F[np.ix_(L, U)] = True
+    return F
+
+
+def choose_split(F, rho=DENSE_RHO, tile_max=TILE_MAX):
+    """Core start c: the longest trailing block at least rho dense, no larger
+    than G * tile_max, and at least one pivot."""
+    n = F.shape[0]
+    c = n - 1
+    for cand in range(n - 1, -1, -1):
+        if F[cand:, cand:].mean() >= rho:
+            c = cand
+    c = max(c, n - G * tile_max)
+    return int(c)
+
+
+def _stage(levels):
+    """Sort task ids by level; return (order, stage_ptr)."""
+    levels = np.asarray(levels, dtype=np.int64)
+    if len(levels) == 0:
+        return np.zeros(0, dtype=np.int64), np.zeros(1, dtype=np.int64)
+    order = np.argsort(levels, kind="stable")
+    nst = int(levels.max()) + 1
+    counts = np.bincount(levels, minlength=nst)
+    return order, np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+
+
+@dataclass
+class LUPlan:
+    n: int
+    nnz: int                 # Jacobian pattern entries
+    perm: np.ndarray         # new -> old
+    c: int                   # first core pivot
+    m: int                   # core size n - c
+    tile: int                # core tile per thread
+    F: np.ndarray            # filled pattern (new ordering)
+    ns: int                  # sparse LU entries (stored in LDS)
+    nnz_lu: int              # all L+U entries of the filled pattern
+    arrays: dict             # flat arrays uploaded to the device
+
+    @property
+    def m_pad(self):
+        return G * self.tile
+
+    @property
+    def lds_zero(self):
+        return self.ns
+
+    @property
+    def lds_vec(self):
+        return self.ns + 1
+
+    @property
+    def lds_len(self):
+        return self.ns + 1 + self.c + self.m_pad
+
+    @property
+    def lu_len(self):
+        """Doubles per cell of a stored factorization: sparse entries, the core
+        tiles and the inverted diagonal blocks."""
+        return self.ns + self.m_pad * self.m_pad + 2 * self.tile * G * G
+
+    def flops(self):
+        """Multiply-adds (+ divisions) of one factorization and one solve on
+        the filled pattern, as a no-pivot sparse LU would do them."""
+        n, F = self.n, self.F
+        fact = 0
+        for k in range(n):
+            nl = int(F[k + 1:, k].sum())
+            nu = int(F[k, k + 1:].sum())
+            fact += nl * nu + nl
+        return fact, int(F.sum())
+
+
+def build_plan(n, rows, cols, tile=None, rho=DENSE_RHO, order=None):
+    """Symbolic analysis of the pattern (rows, cols) (original numbering,
+    diagonal included).  ``order`` overrides the ordering (perm[new] = old)."""
+    rows = np.asarray(rows, dtype=np.int64)
+    cols = np.asarray(cols, dtype=np.int64)
+    nnz = len(rows)
+    perm = min_degree_order(n, rows, cols) if order is None else np.asarray(order, dtype=np.int64)
+    inv = np.empty(n, dtype=np.int64)
+    inv[perm] = np.arange(n)
+    rn, cn = inv[rows], inv[cols]
+    F = symbolic_fill(n, rn, cn)
+    c = choose_split(F, rho)
+    m = n - c
+    if tile is None:
+        tile = max(1, -(-m // G))
+    if G * tile < m:
+        raise ValueError(f"core of {m} rows does not fit tile {tile}")
+    if tile > TILE_MAX:
+        raise ValueError(f"tile {tile} above the supported {TILE_MAX}")
+    m_pad = G * tile
+
+    # ---- sparse storage: entries with min(i, j) < c, row-major
+    sp_index = -np.ones((n, n), dtype=np.int64)
+    si, sj = np.nonzero(F)
+    keep = np.minimum(si, sj) < c
+    si, sj = si[keep], sj[keep]
+    ns = len(si)
+    sp_index[si, sj] = np.arange(ns)
+    ZERO = ns
+    VEC = ns + 1
+    if ns + 1 + c + m_pad >= 65535:
+        raise ValueError("LDS index space exceeds 16 bits")
+
+    # ---- assembly sources.  Entry value = -J[src] (+ alpha*mass[row] on the
+    # diagonal).  src = -1: structural fill (0); for the core, src = -2 marks
+    # an identity padding diagonal.
+    jac_of = -np.ones((n, n), dtype=np.int64)
+    jac_of[rn, cn] = np.arange(nnz)
+    sp_src = jac_of[si, sj].astype(np.int32)
+    sp_diag = np.where(si == sj, perm[si], -1).astype(np.int32)     # old row for mass
+    sp_row = perm[si].astype(np.int32)                               # old row (alg-only mask)
+
+    nslot = tile * tile
+    core_src = np.full((nslot, G * G), -1, dtype=np.int32)
+    core_diag = np.full((nslot, G * G), -1, dtype=np.int32)
+    core_row = np.full((nslot, G * G), -1, dtype=np.int32)
+    for ti in range(tile):
+        for tj in range(tile):
+            s = ti * tile + tj
+            for tr in range(G):
+                for tc in range(G):
+                    t = tr * G + tc
+                    ci, cj = ti * G + tr, tj * G + tc
+                    if ci >= m or cj >= m:
+                        if ci == cj:
+                            core_src[s, t] = -2
+                        continue
+                    i, j = c + ci, c + cj
+                    core_src[s, t] = jac_of[i, j]
+                    core_row[s, t] = perm[i]
+                    if i == j:
+                        core_diag[s, t] = perm[i]
+
+    # ---- sparse factorization tasks (dot-product form)
+    tgt, pstart, pcount, div, level = [], [], [], [], []
+    pairs_a, pairs_b = [], []
+    lev = np.zeros(ns, dtype=np.int64)          # level at which entry is final
+    # process in an order where inputs precede outputs: by min(i, j), then i, j
+    order_e = np.lexsort((sj, si, np.minimum(si, sj)))
+    for e in order_e:
+        i, j = int(si[e]), int(sj[e])
+        lo = min(i, j)
+        ps = [p for p in range(lo) if F[i, p] and F[p, j]]
+        L = 0
+        start = len(pairs_a)
+        for p in ps:
+            a, b = sp_index[i, p], sp_index[p, j]
+            pairs_a.append(a)
+            pairs_b.append(b)
+            L = max(L, lev[a] + 1, lev[b] + 1)
+        d = -1
+        if i > j:
+            d = sp_index[j, j]
+            L = max(L, lev[d] + 1)
+        lev[e] = L
+        if not ps and d < 0:
+            continue            # nothing to compute: value is the assembled one
+        tgt.append(e)
+        pstart.append(start)
+        pcount.append(len(ps))
+        div.append(d)
+        level.append(L - 1 if L > 0 else 0)
+    # levels: a task at level L reads entries final at < L.  Shift so that
+    # tasks whose inputs are all assembled values run in stage 0.
+    fo, fptr = _stage(level)
+    f_tasks = _pack_tasks(np.asarray(tgt)[fo] if len(tgt) else [], np.asarray(pstart)[fo] if len(tgt) else [],
+                          np.asarray(pcount)[fo] if len(tgt) else [], np.asarray(div)[fo] if len(tgt) else [])
+    f_pairs = _pack_pairs(pairs_a, pairs_b)
+
+    # ---- core Schur lists: A_ij -= sum_{p<c} L_ip U_pj, per (slot, thread),
+    # padded per slot to the longest list (pad pairs read the zero entry).
+    lists = {}
+    maxlen = np.zeros(nslot, dtype=np.int64)
+    for ti in range(tile):
+        for tj in range(tile):
+            s = ti * tile + tj
+            for tr in range(G):
+                for tc in range(G):
+                    ci, cj = ti * G + tr, tj * G + tc
+                    if ci >= m or cj >= m:
+                        continue
+                    i, j = c + ci, c + cj
+                    if not F[i, j]:
+                        continue
+                    ps = [p for p in range(c) if F[i, p] and F[p, j]]
+                    if ps:
+                        lists[(s, tr * G + tc)] = [(sp_index[i, p], sp_index[p, j]) for p in ps]
+                        maxlen[s] = max(maxlen[s], len(ps))
+    schur_ptr = np.concatenate([[0], np.cumsum(maxlen)]).astype(np.int32)
+    schur = np.full((int(schur_ptr[-1]), G * G), (ZERO << 16) | ZERO, dtype=np.uint32)
+    for (s, t), lst in lists.items():
+        for q, (a, b) in enumerate(lst):
+            schur[schur_ptr[s] + q, t] = (int(a) << 16) | int(b)
+
+    # ---- solve tasks.  Forward (unit L): v_i -= sum_{p<min(i,c)} L_ip v_p.
+    ftgt, fst, fcnt, flev = [], [], [], []
+    spa, spb = [], []
+    vlev = np.zeros(n, dtype=np.int64)
+    for i in range(n):
+        ps = [p for p in range(min(i, c)) if F[i, p]]
+        if not ps:
+            continue
+        start = len(spa)
+        L = 0
+        for p in ps:
+            spa.append(sp_index[i, p])
+            spb.append(VEC + p)
+            L = max(L, vlev[p] + 1)
+        vlev[i] = L
+        ftgt.append(VEC + i)
+        fst.append(start)
+        fcnt.append(len(ps))
+        flev.append(L - 1)
+    fo2, sfptr = _stage(flev)
+    sf_tasks = _pack_tasks(np.asarray(ftgt)[fo2] if ftgt else [], np.asarray(fst)[fo2] if ftgt else [],
+                           np.asarray(fcnt)[fo2] if ftgt else [], -np.ones(len(ftgt), dtype=np.int64))
+    # Backward over the sparse rows, after the core: v_i = (v_i - sum_{j>i} U_ij v_j) / U_ii.
+    btgt, bst, bcnt, bdiv, blev = [], [], [], [], []
+    ulev = np.zeros(n, dtype=np.int64)
+    for i in range(c - 1, -1, -1):
+        js = [j for j in range(i + 1, n) if F[i, j]]
+        start = len(spa)
+        L = 0
+        for j in js:
+            spa.append(sp_index[i, j])
+            spb.append(VEC + j)
+            L = max(L, ulev[j])          # core entries are final before stage 0
+        ulev[i] = L + 1
+        btgt.append(VEC + i)
+        bst.append(start)
+        bcnt.append(len(js))
+        bdiv.append(sp_index[i, i])
+        blev.append(L)
+    bo, sbptr = _stage(blev)
+    sb_tasks = _pack_tasks(np.asarray(btgt)[bo] if btgt else [], np.asarray(bst)[bo] if btgt else [],
+                           np.asarray(bcnt)[bo] if btgt else [], np.asarray(bdiv)[bo] if btgt else [])
+    s_pairs = _pack_pairs(spa, spb)
+
+    arrays = dict(
+        sp_src=sp_src, sp_diag=sp_diag, sp_row=sp_row,
+        core_src=core_src.ravel(), core_diag=core_diag.ravel(), core_row=core_row.ravel(),
+        f_tasks=f_tasks, f_ptr=fptr.astype(np.int32), f_pairs=f_pairs,
+        schur_ptr=schur_ptr, schur=schur.ravel(),
+        sf_tasks=sf_tasks, sf_ptr=sfptr.astype(np.int32),
+        sb_tasks=sb_tasks, sb_ptr=sbptr.astype(np.int32),
+        s_pairs=s_pairs,
+        perm=perm.astype(np.int32),
+    )
+    return LUPlan(n=n, nnz=nnz, perm=perm, c=c, m=m, tile=tile, F=F, ns=ns,
+                  nnz_lu=int(F.sum()), arrays=arrays)
+
+
+def _pack_tasks(tgt, start, count, div):
+    """Task = 4 x int32: target LDS index, first pair, pair count, divisor LDS
+    index (-1: none)."""
+    k = len(tgt)
+    out = np.zeros((k, 4), dtype=np.int32)
+    if k:
+        out[:, 0] = tgt
+        out[:, 1] = start
+        out[:, 2] = count
+        out[:, 3] = div
+    return out.ravel()
+
+
+def _pack_pairs(a, b):
+    a = np.asarray(a, dtype=np.int64)
+    b = np.asarray(b, dtype=np.int64)
+    return ((a << 16) | b).astype(np.uint32)
+
+
+# ---------------------------------------------------------------------------
+# A CPU model of exactly what the device does with a plan: used by the tests
+# to check a plan before any kernel runs, never by the product path.
+# ---------------------------------------------------------------------------
+
+def emulate_factor(plan: LUPlan, A_old: np.ndarray):
+    """Factor one dense matrix (original numbering) the way the device does.
+    Returns (lds, core) with lds the sparse entries and core the m_pad^2
+    dense core after no-pivot LU."""
+    n, c, m, mp = plan.n, plan.c, plan.m, plan.m_pad
+    p = plan.perm
+    A = A_old[np.ix_(p, p)]
+    ar = plan.arrays
+    lds = np.zeros(plan.lds_len)
+    si, sj = np.nonzero(plan.F)
+    keep = np.minimum(si, sj) < c
+    si, sj = si[keep], sj[keep]
+    lds[: plan.ns] = A[si, sj]
+    tasks = ar["f_tasks"].reshape(-1, 4)
+    pa = (ar["f_pairs"] >> 16).astype(np.int64)
+    pb = (ar["f_pairs"] & 0xFFFF).astype(np.int64)
+    ptr = ar["f_ptr"]
+    for s in range(len(ptr) - 1):
+        new = {}
+        for t in range(ptr[s], ptr[s + 1]):
+            tg, st, cnt, d = tasks[t]
+            acc = lds[tg]
+            for q in range(st, st + cnt):
+                acc -= lds[pa[q]] * lds[pb[q]]
+            if d >= 0:
+                acc = acc / lds[d]
+            new[tg] = acc
+        for tg, v in new.items():
+            lds[tg] = v
+    core = np.eye(mp)
+    core[:m, :m] = A[c:, c:] * plan.F[c:, c:]
+    # Schur lists
+    sp = ar["schur_ptr"]
+    sch = ar["schur"].reshape(-1, G * G)
+    tile = plan.tile
+    for ti in range(tile):
+        for tj in range(tile):
+            s = ti * tile + tj
+            for t in range(G * G):
+                tr, tc = divmod(t, G)
+                ci, cj = ti * G + tr, tj * G + tc
+                acc = core[ci, cj]
+                for q in range(sp[s], sp[s + 1]):
+                    w = int(sch[q, t])
+                    acc -= lds[w >> 16] * lds[w & 0xFFFF]
+                core[ci, cj] = acc
+    for k in range(mp):
+        core[k + 1:, k] /= core[k, k]
+        core[k + 1:, k + 1:] -= np.outer(core[k + 1:, k], core[k, k + 1:])
+    return lds, core
+
+
+def emulate_solve(plan: LUPlan, lds, core, b_old):
+    n, c, m, mp = plan.n, plan.c, plan.m, plan.m_pad
+    p = plan.perm
+    lds = lds.copy()
+    V = plan.lds_vec
+    lds[V: V + n] = b_old[p]
+    lds[V + n:] = 0.0
+    ar = plan.arrays
+    pa = (ar["s_pairs"] >> 16).astype(np.int64)
+    pb = (ar["s_pairs"] & 0xFFFF).astype(np.int64)
+
+    def run(tasks, ptr):
+        tasks = tasks.reshape(-1, 4)
+        for s in range(len(ptr) - 1):
+            new = {}
+            for t in range(ptr[s], ptr[s + 1]):
+                tg, st, cnt, d = tasks[t]
+                acc = lds[tg]
+                for q in range(st, st + cnt):
+                    acc -= lds[pa[q]] * lds[pb[q]]
+                if d >= 0:
+                    acc = acc / lds[d]
+                new[tg] = acc
+            for tg, v in new.items():
+                lds[tg] = v
+
+    run(ar["sf_tasks"], ar["sf_ptr"])
+    y = lds[V + c: V + c + mp].copy()
+    Lc = np.tril(core, -1) + np.eye(mp)
+    Uc = np.triu(core)
+    y = np.linalg.solve(Lc, y)
+    y = np.linalg.solve(Uc, y)
+    lds[V + c: V + c + mp] = y
+    run(ar["sb_tasks"], ar["sb_ptr"])
+    x = np.empty(n)
+    x[p] = lds[V: V + n]
+    return x
