@@ -1,0 +1,525 @@
+// Plain-path kernels and the C ABI of the integrator (include/fastode_hip.h).
+//
+// The plain path keeps the reference's call structure: one launch per
+// protocol operation (rhs, jac, factor, solve, stage arithmetic, norm +
+// control) on one group of cells, with the controller state on the device.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "engine.hpp"
+
+using namespace fode;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(const std::string& msg) {
+  g_err = msg;
+  return -1;
+}
+
+#define HIP_OK(expr)                                                             \
+  do {                                                                           \
+    hipError_t e_ = (expr);                                                      \
+    if (e_ != hipSuccess) return fail(std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <class T>
+int upload(const T* host, size_t count, T** dev, std::vector<void*>& owned) {
+  *dev = nullptr;
+  if (count == 0) return 0;
+  HIP_OK(hipMalloc((void**)dev, count * sizeof(T)));
+  owned.push_back(*dev);
+  HIP_OK(hipMemcpy(*dev, host, count * sizeof(T), hipMemcpyHostToDevice));
+  return 0;
+}
+
+struct MechHandle {
+  MechDev d;
+  std::vector<void*> owned;
+};
+
+struct PlanHandle {
+  PlanDev d;
+  std::vector<void*> owned;
+};
+
+hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// ----------------------------------------------------------------- kernels
+__global__ void k_rhs(MechDev M, int C, const double* __restrict__ Y, const double* __restrict__ k,
+                      double* __restrict__ F) {
+  const long total = (long)C * M.n;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total;
+       idx += (long)gridDim.x * blockDim.x) {
+    const int c = (int)(idx / M.n), i = (int)(idx % M.n);
+    const double* y = Y + (size_t)c * M.n;
+    const double* kc = k + (size_t)c * M.R;
+    double f;
+    const int ci = M.con_of_row[i];
+    if (ci >= 0) {
+      f = 0.0;
+      const double* cf = M.con_coef + (size_t)ci * M.n;
+      for (int s = 0; s < M.n; ++s) f = f + cf[s] * y[s];
+      f = f - M.con_total[ci];
+    } else {
+      f = 0.0;
+      for (int q = M.rhs_ptr[i]; q < M.rhs_ptr[i + 1]; ++q)
+        f = f + rate(M, y, kc, M.rhs_rxn[q]) * M.rhs_coef[q];
+    }
+    F[idx] = f;
+  }
+}
+
+__global__ void k_jac(MechDev M, int C, const double* __restrict__ Y, const double* __restrict__ k,
+                      double* __restrict__ J) {
+  const long total = (long)C * M.nnz;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total;
+       idx += (long)gridDim.x * blockDim.x) {
+    const int c = (int)(idx / M.nnz), e = (int)(idx % M.nnz);
+    const double* y = Y + (size_t)c * M.n;
+    const double* kc = k + (size_t)c * M.R;
+    double v = 0.0;
+    for (int q = M.jac_ptr[e]; q < M.jac_ptr[e + 1]; ++q) {
+      const int jp = M.jac_src[q];
+      v = v + M.jac_coef[q] * drate(M, y, kc, jp / M.P, jp % M.P);
+    }
+    J[idx] = v + M.jac_const[e];
+  }
+}
+
+struct JGlobal {
+  const double* J;
+  __device__ double operator()(int e) const { return J[e]; }
+};
+
+template <int TILE>
+__global__ __launch_bounds__(NT) void k_factor(PlanDev P, const double* __restrict__ alpha_p,
+                                               const double* __restrict__ mass,
+                                               const double* __restrict__ J, int alg_only,
+                                               double* __restrict__ LU) {
+  extern __shared__ double smem[];
+  const EngineLds L = carve(smem, P);
+  const int cell = blockIdx.x, tid = threadIdx.x;
+  const double alpha = *alpha_p;
+  double a[TILE][TILE], dLt[TILE], dUt[TILE];
+  assemble<TILE>(P, JGlobal{J + (size_t)cell * P.nnz}, alpha, mass, alg_only != 0, L.lds, a);
+  factor_cell<TILE>(P, L, a, dLt, dUt);
+  double* out = LU + (size_t)cell * P.lu_len;
+  for (int s = tid; s < P.ns; s += NT) out[s] = L.lds[s];
+  double* core = out + P.ns;
+#pragma unroll
+  for (int ti = 0; ti < TILE; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < TILE; ++tj) core[(ti * TILE + tj) * NT + tid] = a[ti][tj];
+  double* dd = core + TILE * TILE * NT;
+#pragma unroll
+  for (int J2 = 0; J2 < TILE; ++J2) {
+    dd[J2 * NT + tid] = dLt[J2];
+    dd[(TILE + J2) * NT + tid] = dUt[J2];
+  }
+}
+
+template <int TILE>
+__global__ __launch_bounds__(NT) void k_solve(PlanDev P, const double* __restrict__ LU,
+                                              const double* B, double* X) {
+  extern __shared__ double smem[];
+  const EngineLds L = carve(smem, P);
+  const int cell = blockIdx.x, tid = threadIdx.x;
+  const double* in = LU + (size_t)cell * P.lu_len;
+  double a[TILE][TILE], dLt[TILE], dUt[TILE];
+  for (int s = tid; s < P.ns; s += NT) L.lds[s] = in[s];
+  if (tid == 0) L.lds[P.ns] = 0.0;
+  const double* core = in + P.ns;
+#pragma unroll
+  for (int ti = 0; ti < TILE; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < TILE; ++tj) a[ti][tj] = core[(ti * TILE + tj) * NT + tid];
+  const double* dd = core + TILE * TILE * NT;
+#pragma unroll
+  for (int J2 = 0; J2 < TILE; ++J2) {
+    dLt[J2] = dd[J2 * NT + tid];
+    dUt[J2] = dd[(TILE + J2) * NT + tid];
+  }
+  double* vec = L.lds + P.ns + 1;
+  const double* b = B + (size_t)cell * P.n;
+  for (int i = tid; i < P.c + P.mpad; i += NT) vec[i] = (i < P.n) ? b[P.perm[i]] : 0.0;
+  solve_cell<TILE>(P, L, a, dLt, dUt);
+  double* x = X + (size_t)cell * P.n;
+  for (int i = tid; i < P.n; i += NT) x[P.perm[i]] = vec[i];
+}
+
+struct KPtrs {
+  const double* k[4];
+};
+
+__global__ void k_stage_point(fode_ctrl_params p, int r, long N, const double* __restrict__ Y,
+                              KPtrs K, double* __restrict__ out) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+    double v = Y[i];
+    for (int s = 0; s < r; ++s) {
+      const double a = p.a[r * 4 + s];
+      if (a != 0.0) v = v + a * K.k[s][i];
+    }
+    out[i] = v;
+  }
+}
+
+__global__ void k_stage_forcing(fode_ctrl_params p, int r, long N, int n,
+                                const fode_group_ctrl* __restrict__ g, const double* __restrict__ F,
+                                KPtrs K, const double* __restrict__ mass, int dae, double* out) {
+  const double H = g->H;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+    double v = F[i];
+    for (int s = 0; s < r; ++s) {
+      const double coef = p.c[r * 4 + s] / H;
+      if (coef != 0.0) {
+        const double ks = dae ? K.k[s][i] * mass[i % n] : K.k[s][i];
+        v = v + coef * ks;
+      }
+    }
+    out[i] = v;
+  }
+}
+
+__global__ void k_stage_final(fode_ctrl_params p, long N, const double* __restrict__ Y, KPtrs K,
+                              double* __restrict__ Ynew, double* __restrict__ E) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+    double y = Y[i], e = 0.0;
+    for (int r = 0; r < p.stages; ++r) {
+      const double kr = K.k[r][i];
+      if (p.m[r] != 0.0) y = y + p.m[r] * kr;
+      if (p.e[r] != 0.0) e = e + p.e[r] * kr;
+    }
+    Ynew[i] = y;
+    E[i] = e;
+  }
+}
+
+__device__ inline void clamp_range(double* Y, long N, int n, const double* mass, int constrained,
+                                   long start, long stride) {
+  for (long i = start; i < N; i += stride) {
+    if (constrained && mass[i % n] != 1.0) continue;
+    Y[i] = fmax(Y[i], 0.0);
+  }
+}
+
+__global__ void k_ctrl_init(fode_ctrl_params p, fode_group_ctrl* g, int ngroups, int begin_later) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ngroups) return;
+  fode_group_ctrl c = g[i];
+  ctrl_init(p, c);
+  if (!begin_later) ctrl_outer(p, c);
+  g[i] = c;
+}
+
+__global__ void k_ctrl_begin(fode_ctrl_params p, fode_group_ctrl* g, int ngroups) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ngroups) return;
+  fode_group_ctrl c = g[i];
+  ctrl_outer(p, c);
+  g[i] = c;
+}
+
+__global__ __launch_bounds__(256) void k_norm_control(fode_ctrl_params p, fode_group_ctrl* g, int C,
+                                                      int n, double* Y, const double* Ynew,
+                                                      const double* E, const double* atol,
+                                                      double rtol, const double* mass) {
+  __shared__ double red[4];
+  __shared__ int flags[2];
+  const long N = (long)C * n;
+  double acc = 0.0;
+  for (long i = threadIdx.x; i < N; i += blockDim.x) {
+    const double ymax = fmax(fabs(Y[i]), fabs(Ynew[i]));
+    const double s = E[i] / (atol[i % n] + rtol * ymax);
+    acc = acc + s * s;
+  }
+  const double sum = block_sum<256>(acc, red);
+  if (threadIdx.x == 0) {
+    const double v = sqrt(sum / (double)N);
+    const double err = isnan(v) ? v : fmax(v, 1.0e-10);
+    fode_group_ctrl c = *g;
+    ctrl_after(p, c, err);
+    *g = c;
+    flags[0] = c.accepted;
+    flags[1] = c.action;
+  }
+  __syncthreads();
+  if (flags[0]) {
+    for (long i = threadIdx.x; i < N; i += blockDim.x) Y[i] = Ynew[i];
+  }
+  if (flags[1] == FODE_FINISH) {
+    __syncthreads();
+    clamp_range(Y, N, n, mass, p.constrained, threadIdx.x, blockDim.x);
+  }
+}
+
+__global__ void k_clamp(long N, int n, double* Y, const double* mass, int constrained) {
+  clamp_range(Y, N, n, mass, constrained, blockIdx.x * (long)blockDim.x + threadIdx.x,
+              (long)gridDim.x * blockDim.x);
+}
+
+int grid_for(long N, int block = 256) {
+  long g = (N + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > 65536) g = 65536;
+  return (int)g;
+}
+
+size_t engine_bytes(const PlanDev& P) {
+  return (size_t)engine_lds_doubles(P.lds_len, P.mpad, P.tile) * sizeof(double);
+}
+
+template <int TILE>
+int launch_factor(const PlanDev& P, int C, const double* alpha, const double* mass, const double* J,
+                  int alg_only, double* LU, hipStream_t s) {
+  const size_t bytes = engine_bytes(P);
+  HIP_OK(hipFuncSetAttribute((const void*)k_factor<TILE>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)bytes));
+  hipLaunchKernelGGL(k_factor<TILE>, dim3(C), dim3(NT), bytes, s, P, alpha, mass, J, alg_only, LU);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+template <int TILE>
+int launch_solve(const PlanDev& P, int C, const double* LU, const double* B, double* X, hipStream_t s) {
+  const size_t bytes = engine_bytes(P);
+  HIP_OK(hipFuncSetAttribute((const void*)k_solve<TILE>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)bytes));
+  hipLaunchKernelGGL(k_solve<TILE>, dim3(C), dim3(NT), bytes, s, P, LU, B, X);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+#define FODE_TILES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10)
+
+KPtrs kptrs(const double* const* K, int count) {
+  KPtrs k{};
+  for (int i = 0; i < 4; ++i) k.k[i] = (i < count && K) ? K[i] : nullptr;
+  return k;
+}
+
+}  // namespace
+
+// =================================================================== C ABI
+extern "C" {
+
+const char* fode_last_error(void) { return g_err.c_str(); }
+
+int fode_device_info(int device, int* lds_per_block, int* cus, int* warp_size) {
+  HIP_OK(hipDeviceGetAttribute(lds_per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+  HIP_OK(hipDeviceGetAttribute(cus, hipDeviceAttributeMultiprocessorCount, device));
+  HIP_OK(hipDeviceGetAttribute(warp_size, hipDeviceAttributeWarpSize, device));
+  return 0;
+}
+
+int fode_mech_create(const fode_mech_desc* d, void** out) {
+  if (!d || !out) return fail("fode_mech_create: null argument");
+  if (d->n <= 0 || d->R < 0 || d->P <= 0 || d->nnz < d->n) return fail("fode_mech_create: bad sizes");
+  auto* h = new MechHandle();
+  MechDev& m = h->d;
+  m.n = d->n; m.R = d->R; m.P = d->P; m.nnz = d->nnz; m.ncon = d->ncon;
+  int rc = 0;
+  rc |= upload(d->react, (size_t)d->R * d->P, (int**)&m.react, h->owned);
+  rc |= upload(d->rhs_ptr, (size_t)d->n + 1, (int**)&m.rhs_ptr, h->owned);
+  rc |= upload(d->rhs_rxn, (size_t)d->rhs_ptr[d->n], (int**)&m.rhs_rxn, h->owned);
+  rc |= upload(d->rhs_coef, (size_t)d->rhs_ptr[d->n], (double**)&m.rhs_coef, h->owned);
+  rc |= upload(d->jac_ptr, (size_t)d->nnz + 1, (int**)&m.jac_ptr, h->owned);
+  rc |= upload(d->jac_src, (size_t)d->jac_ptr[d->nnz], (int**)&m.jac_src, h->owned);
+  rc |= upload(d->jac_coef, (size_t)d->jac_ptr[d->nnz], (double**)&m.jac_coef, h->owned);
+  rc |= upload(d->jac_const, (size_t)d->nnz, (double**)&m.jac_const, h->owned);
+  std::vector<int> con_of(d->n, -1);
+  for (int c = 0; c < d->ncon; ++c) {
+    if (d->con_rows[c] < 0 || d->con_rows[c] >= d->n) { delete h; return fail("fode_mech_create: bad constraint row"); }
+    con_of[d->con_rows[c]] = c;
+  }
+  rc |= upload(con_of.data(), con_of.size(), (int**)&m.con_of_row, h->owned);
+  if (d->ncon > 0) {
+    rc |= upload(d->con_rows, (size_t)d->ncon, (int**)&m.con_rows, h->owned);
+    rc |= upload(d->con_coef, (size_t)d->ncon * d->n, (double**)&m.con_coef, h->owned);
+    rc |= upload(d->con_total, (size_t)d->ncon, (double**)&m.con_total, h->owned);
+  }
+  if (rc) {
+    for (void* p : h->owned) (void)hipFree(p);
+    delete h;
+    return -1;
+  }
+  *out = h;
+  return 0;
+}
+
+int fode_mech_destroy(void* mech) {
+  auto* h = static_cast<MechHandle*>(mech);
+  if (!h) return 0;
+  for (void* p : h->owned) (void)hipFree(p);
+  delete h;
+  return 0;
+}
+
+int fode_plan_create(const fode_plan_desc* d, void** out, int64_t* lu_len) {
+  if (!d || !out) return fail("fode_plan_create: null argument");
+  if (d->tile < 1 || d->tile > 10) return fail("fode_plan_create: tile must be 1..10");
+  if (d->m > G * d->tile || d->c + d->m != d->n) return fail("fode_plan_create: bad core split");
+  auto* h = new PlanHandle();
+  PlanDev& p = h->d;
+  p.n = d->n; p.c = d->c; p.m = d->m; p.tile = d->tile; p.ns = d->ns; p.nnz = d->nnz;
+  p.mpad = G * d->tile;
+  p.lds_len = d->ns + 1 + d->c + p.mpad;
+  p.lu_len = d->ns + p.mpad * p.mpad + 2 * d->tile * NT;
+  p.f_nst = d->f_nst; p.sf_nst = d->sf_nst; p.sb_nst = d->sb_nst;
+  const size_t nslot = (size_t)d->tile * d->tile * NT;
+  int rc = 0;
+  rc |= upload(d->sp_src, (size_t)d->ns, (int**)&p.sp_src, h->owned);
+  rc |= upload(d->sp_diag, (size_t)d->ns, (int**)&p.sp_diag, h->owned);
+  rc |= upload(d->sp_row, (size_t)d->ns, (int**)&p.sp_row, h->owned);
+  rc |= upload(d->core_src, nslot, (int**)&p.core_src, h->owned);
+  rc |= upload(d->core_diag, nslot, (int**)&p.core_diag, h->owned);
+  rc |= upload(d->core_row, nslot, (int**)&p.core_row, h->owned);
+  rc |= upload((const int4*)d->f_tasks, (size_t)d->f_ntasks, (int4**)&p.f_tasks, h->owned);
+  rc |= upload(d->f_ptr, (size_t)d->f_nst + 1, (int**)&p.f_ptr, h->owned);
+  rc |= upload(d->f_pairs, (size_t)d->f_npairs, (uint32_t**)&p.f_pairs, h->owned);
+  rc |= upload(d->schur_ptr, (size_t)d->tile * d->tile + 1, (int**)&p.schur_ptr, h->owned);
+  rc |= upload(d->schur, (size_t)d->schur_rows * NT, (uint32_t**)&p.schur, h->owned);
+  rc |= upload((const int4*)d->sf_tasks, (size_t)d->sf_ntasks, (int4**)&p.sf_tasks, h->owned);
+  rc |= upload(d->sf_ptr, (size_t)d->sf_nst + 1, (int**)&p.sf_ptr, h->owned);
+  rc |= upload((const int4*)d->sb_tasks, (size_t)d->sb_ntasks, (int4**)&p.sb_tasks, h->owned);
+  rc |= upload(d->sb_ptr, (size_t)d->sb_nst + 1, (int**)&p.sb_ptr, h->owned);
+  rc |= upload(d->s_pairs, (size_t)d->s_npairs, (uint32_t**)&p.s_pairs, h->owned);
+  rc |= upload(d->perm, (size_t)d->n, (int**)&p.perm, h->owned);
+  if (rc) {
+    for (void* q : h->owned) (void)hipFree(q);
+    delete h;
+    return -1;
+  }
+  if (lu_len) *lu_len = p.lu_len;
+  *out = h;
+  return 0;
+}
+
+int fode_plan_destroy(void* plan) {
+  auto* h = static_cast<PlanHandle*>(plan);
+  if (!h) return 0;
+  for (void* p : h->owned) (void)hipFree(p);
+  delete h;
+  return 0;
+}
+
+int fode_rhs(void* mech, int C, const double* Y, const double* k, double* F, void* stream) {
+  if (!mech || C < 0) return fail("fode_rhs: bad argument");
+  if (C == 0) return 0;
+  const MechDev& M = static_cast<MechHandle*>(mech)->d;
+  hipLaunchKernelGGL(k_rhs, dim3(grid_for((long)C * M.n)), dim3(256), 0, S(stream), M, C, Y, k, F);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+int fode_jac(void* mech, int C, const double* Y, const double* k, double* J, void* stream) {
+  if (!mech || C < 0) return fail("fode_jac: bad argument");
+  if (C == 0) return 0;
+  const MechDev& M = static_cast<MechHandle*>(mech)->d;
+  hipLaunchKernelGGL(k_jac, dim3(grid_for((long)C * M.nnz)), dim3(256), 0, S(stream), M, C, Y, k, J);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+int fode_factor(void* plan, int C, const double* alpha, const double* mass, const double* J,
+                int alg_only, double* LU, void* stream) {
+  if (!plan || C < 0 || !alpha) return fail("fode_factor: bad argument");
+  if (C == 0) return 0;
+  const PlanDev& P = static_cast<PlanHandle*>(plan)->d;
+  switch (P.tile) {
+#define X(T) case T: return launch_factor<T>(P, C, alpha, mass, J, alg_only, LU, S(stream));
+    FODE_TILES(X)
+#undef X
+  }
+  return fail("fode_factor: unsupported tile");
+}
+
+int fode_solve(void* plan, int C, const double* LU, const double* B, double* X, void* stream) {
+  if (!plan || C < 0) return fail("fode_solve: bad argument");
+  if (C == 0) return 0;
+  const PlanDev& P = static_cast<PlanHandle*>(plan)->d;
+  switch (P.tile) {
+#define X(T) case T: return launch_solve<T>(P, C, LU, B, X, S(stream));
+    FODE_TILES(X)
+#undef X
+  }
+  return fail("fode_solve: unsupported tile");
+}
+
+int fode_ctrl_init(const fode_ctrl_params* p, fode_group_ctrl* g, int ngroups, int begin_later,
+                   void* stream) {
+  if (!p || !g || ngroups < 0) return fail("fode_ctrl_init: bad argument");
+  if (ngroups == 0) return 0;
+  hipLaunchKernelGGL(k_ctrl_init, dim3((ngroups + 255) / 256), dim3(256), 0, S(stream), *p, g, ngroups,
+                     begin_later);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+int fode_ctrl_begin(const fode_ctrl_params* p, fode_group_ctrl* g, int ngroups, void* stream) {
+  if (!p || !g || ngroups < 0) return fail("fode_ctrl_begin: bad argument");
+  if (ngroups == 0) return 0;
+  hipLaunchKernelGGL(k_ctrl_begin, dim3((ngroups + 255) / 256), dim3(256), 0, S(stream), *p, g, ngroups);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+int fode_stage_point(const fode_ctrl_params* p, int r, int C, int n, const double* Y,
+                     const double* const* K, double* out, void* stream) {
+  if (!p || r < 0 || r >= 4) return fail("fode_stage_point: bad argument");
+  const long N = (long)C * n;
+  if (N == 0) return 0;
+  hipLaunchKernelGGL(k_stage_point, dim3(grid_for(N)), dim3(256), 0, S(stream), *p, r, N, Y,
+                     kptrs(K, r), out);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+int fode_stage_forcing(const fode_ctrl_params* p, int r, int C, int n, const fode_group_ctrl* g,
+                       const double* F, const double* const* K, const double* mass, int dae,
+                       double* out, void* stream) {
+  if (!p || !g || r < 0 || r >= 4) return fail("fode_stage_forcing: bad argument");
+  const long N = (long)C * n;
+  if (N == 0) return 0;
+  hipLaunchKernelGGL(k_stage_forcing, dim3(grid_for(N)), dim3(256), 0, S(stream), *p, r, N, n, g, F,
+                     kptrs(K, r), mass, dae, out);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+int fode_stage_final(const fode_ctrl_params* p, int C, int n, const double* Y, const double* const* K,
+                     double* Ynew, double* E, void* stream) {
+  if (!p || p->stages < 1 || p->stages > 4) return fail("fode_stage_final: bad argument");
+  const long N = (long)C * n;
+  if (N == 0) return 0;
+  hipLaunchKernelGGL(k_stage_final, dim3(grid_for(N)), dim3(256), 0, S(stream), *p, N, Y,
+                     kptrs(K, p->stages), Ynew, E);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+int fode_norm_control(const fode_ctrl_params* p, fode_group_ctrl* g, int C, int n, double* Y,
+                      const double* Ynew, const double* E, const double* atol, double rtol,
+                      const double* mass, void* stream) {
+  if (!p || !g || C <= 0 || n <= 0) return fail("fode_norm_control: bad argument");
+  hipLaunchKernelGGL(k_norm_control, dim3(1), dim3(256), 0, S(stream), *p, g, C, n, Y, Ynew, E, atol,
+                     rtol, mass);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+int fode_clamp(int C, int n, double* Y, const double* mass, int constrained, void* stream) {
+  const long N = (long)C * n;
+  if (N == 0) return 0;
+  hipLaunchKernelGGL(k_clamp, dim3(grid_for(N)), dim3(256), 0, S(stream), N, n, Y, mass, constrained);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+}  // extern "C"

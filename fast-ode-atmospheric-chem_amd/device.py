@@ -1,0 +1,114 @@
+"""Device-resident mechanism tables and LU plans (created once, cached).
+
+A ``DeviceMechanism`` uploads a ``Mechanism``'s gather tables through
+``fode_mech_create``; a ``DevicePlan`` runs the host symbolic analysis of a
+pattern (``symbolic.build_plan``) and uploads it through ``fode_plan_create``.
+Both are cached on the host object they come from, as the reference shares one
+mechanism and pattern across the per-interval systems (fastode/ts1.py:215).
+"""
+from __future__ import annotations
+
+import ctypes as ct
+
+import numpy as np
+
+from . import _lib as L
+from .symbolic import build_plan
+
+
+def _i32(a):
+    return np.ascontiguousarray(np.asarray(a), dtype=np.int32)
+
+
+def _f64(a):
+    return np.ascontiguousarray(np.asarray(a), dtype=np.float64)
+
+
+def _u32(a):
+    return np.ascontiguousarray(np.asarray(a), dtype=np.uint32)
+
+
+def _addr(a):
+    return a.ctypes.data_as(ct.c_void_p) if a.size else None
+
+
+class DeviceMechanism:
+    def __init__(self, mech):
+        self.mech = mech
+        arrs = dict(
+            react=_i32(mech.react), rhs_ptr=_i32(mech.rhs_ptr), rhs_rxn=_i32(mech.rhs_rxn),
+            rhs_coef=_f64(mech.rhs_coef), jac_ptr=_i32(mech.jac_ptr), jac_src=_i32(mech.jac_src),
+            jac_coef=_f64(mech.jac_coef), jac_const=_f64(mech.jac_const),
+            con_rows=_i32(mech.constraint_rows), con_coef=_f64(mech.constraint_coef),
+            con_total=_f64(mech.constraint_total))
+        d = L.MechDesc(n=mech.n, R=mech.R, P=mech.P, nnz=mech.nnz, ncon=len(mech.constraint_rows),
+                       **{k: _addr(v) for k, v in arrs.items()})
+        h = ct.c_void_p()
+        L.check(L.lib.fode_mech_create(ct.byref(d), ct.byref(h)), "fode_mech_create")
+        self.handle = h
+        self._keep = arrs
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h:
+            L.lib.fode_mech_destroy(h)
+            self.handle = None
+
+
+class DevicePlan:
+    def __init__(self, n, rows, cols, **plan_kw):
+        plan = build_plan(n, rows, cols, **plan_kw)
+        self.plan = plan
+        a = plan.arrays
+        arrs = {k: (_u32(v) if k in ("f_pairs", "schur", "s_pairs") else _i32(v)) for k, v in a.items()}
+        d = L.PlanDesc(
+            n=plan.n, c=plan.c, m=plan.m, tile=plan.tile, ns=plan.ns, nnz=plan.nnz,
+            f_ntasks=len(a["f_tasks"]) // 4, f_nst=len(a["f_ptr"]) - 1, f_npairs=len(a["f_pairs"]),
+            schur_rows=int(a["schur_ptr"][-1]),
+            sf_ntasks=len(a["sf_tasks"]) // 4, sf_nst=len(a["sf_ptr"]) - 1,
+            sb_ntasks=len(a["sb_tasks"]) // 4, sb_nst=len(a["sb_ptr"]) - 1,
+            s_npairs=len(a["s_pairs"]),
+            **{k: _addr(v) for k, v in arrs.items()})
+        h = ct.c_void_p()
+        lu_len = ct.c_int64()
+        L.check(L.lib.fode_plan_create(ct.byref(d), ct.byref(h), ct.byref(lu_len)), "fode_plan_create")
+        self.handle = h
+        self.lu_len = lu_len.value
+        self._keep = arrs
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h:
+            L.lib.fode_plan_destroy(h)
+            self.handle = None
+
+
+def device_mechanism(mech) -> DeviceMechanism:
+    dm = getattr(mech, "_device", None)
+    if dm is None:
+        dm = DeviceMechanism(mech)
+        mech._device = dm
+    return dm
+
+
+def mechanism_plan(mech) -> DevicePlan:
+    dp = getattr(mech, "_plan", None)
+    if dp is None:
+        dp = DevicePlan(mech.n, mech.rows, mech.cols)
+        mech._plan = dp
+    return dp
+
+
+_DENSE_PLANS: dict = {}
+
+
+def dense_plan(n) -> DevicePlan:
+    """Plan of a fully dense n x n pattern in row-major entry order, so that a
+    dense Jacobian (C, n, n) is read as nnz = n*n values per cell."""
+    dp = _DENSE_PLANS.get(n)
+    if dp is None:
+        rows = np.repeat(np.arange(n), n)
+        cols = np.tile(np.arange(n), n)
+        dp = DevicePlan(n, rows, cols, order=np.arange(n))
+        _DENSE_PLANS[n] = dp
+    return dp

@@ -1,0 +1,290 @@
+"""The plain drop-in: ``Algorithm(ROS3|RODAS3).solve(system, Y, dt, atol, rtol,
+counts, memory)`` on one group, with the reference's signature, return values
+and status strings (fastode/solvers.py:270-415).
+
+The host keeps the reference's call structure (one protocol call per
+operation, fastode/solvers.py:330-411) while every number lives on the
+device: the stage arithmetic, the group error norm and the accept/reject
+controller with its H211b memory and step quantization run in HIP kernels
+(``fode_stage_*``, ``fode_norm_control``), and the host only reads the
+group's small control block after each attempt to learn what comes next.
+``project`` (fastode/solvers.py:100-178) runs its Newton iteration on the
+device system; its host decisions read device reductions.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import math
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from .counts import Counts, Memory
+from .system import F64, as_device, device
+from .tableaus import ROS3, Tableau
+
+
+def quantize(h, interval):
+    """Host copy of the step rounding (fastode/solvers.py:46-51), for callers
+    and tests; the solver rounds on the device."""
+    if h >= interval:
+        return interval
+    return interval / max(math.ceil(interval / h - 1.0e-9), 1)
+
+
+def ctrl_params(alg, tab: Tableau, constrained: bool) -> L.CtrlParams:
+    S = tab.stages
+    p = L.CtrlParams()
+    p.b, p.safety, p.fac_min, p.fac_max = alg.b, alg.safety, alg.fac_min, alg.fac_max
+    p.k_exp, p.gamma = tab.k, tab.gamma
+    p.h211b = int(alg.controller == "h211b")
+    p.carry, p.quantize, p.carry_memory = int(alg.carry), int(alg.quantize), int(alg.carry_memory)
+    p.max_steps = int(alg.max_steps)
+    p.stages = S
+    nf = tab.new_forcing
+    p.n_new_forcing = int(sum(nf[1:]))
+    p.constrained = int(constrained)
+    a = np.zeros((4, 4))
+    c = np.zeros((4, 4))
+    a[:S, :S] = tab.a
+    c[:S, :S] = tab.c
+    for i in range(16):
+        p.a[i] = a.ravel()[i]
+        p.c[i] = c.ravel()[i]
+    for i in range(S):
+        p.m[i] = tab.m[i]
+        p.e[i] = tab.e[i]
+        p.new_forcing[i] = int(nf[i])
+    return p
+
+
+class GroupControl:
+    """One group's device control block (fode_group_ctrl)."""
+
+    def __init__(self, dt, memory: Memory):
+        rec = np.zeros(1, dtype=L.CTRL_DTYPE)
+        rec["dt"] = dt
+        rec["h_next"], rec["err_prev"], rec["h_prev"] = memory.h_next, memory.err_prev, memory.h_prev
+        self.dev = torch.from_numpy(rec.view(np.int64).copy()).to(device())
+        self.ptr = ct.c_void_p(self.dev.data_ptr())
+
+    def read(self):
+        return self.dev.cpu().numpy().view(L.CTRL_DTYPE)[0]
+
+    def alpha(self):
+        """0-dim view of the device alpha = 1/(H gamma) of the next attempt."""
+        return self.dev.view(F64)[L.ALPHA_OFFSET // 8: L.ALPHA_OFFSET // 8 + 1]
+
+
+def stages(system, tab, params, Y, F0, LU, ctrl, K, work):
+    """One Rosenbrock attempt (fastode/solvers.py:54-86) on the device; H is
+    read by the kernels from the control block."""
+    C, n = Y.shape
+    s = L.stream_handle()
+    dae = int(system.has_constraints)
+    nf = tab.new_forcing
+    forcing = F0
+    for r in range(tab.stages):
+        if r > 0 and nf[r]:
+            Yr = work["Yr"]
+            L.check(L.lib.fode_stage_point(ct.byref(params), r, C, n, L.ptr(Y), L.ptr_array(K[:r]),
+                                           L.ptr(Yr), s), "fode_stage_point")
+            forcing = system.rhs(Yr)
+        rhs = work["rhs"]
+        L.check(L.lib.fode_stage_forcing(ct.byref(params), r, C, n, ctrl.ptr, L.ptr(forcing),
+                                         L.ptr_array(K[:r]), L.ptr(system.mass_dev), dae, L.ptr(rhs), s),
+                "fode_stage_forcing")
+        LU.solve(rhs, out=K[r])
+    Ynew, E = work["Ynew"], work["E"]
+    L.check(L.lib.fode_stage_final(ct.byref(params), C, n, L.ptr(Y), L.ptr_array(K[: tab.stages]),
+                                   L.ptr(Ynew), L.ptr(E), s), "fode_stage_final")
+    return Ynew, E
+
+
+def clamp(system, Y):
+    """max(Y, 0), differential columns only with constraints
+    (fastode/solvers.py:89-97); in place on the device tensor."""
+    C, n = Y.shape
+    L.check(L.lib.fode_clamp(C, n, L.ptr(Y), L.ptr(system.mass_dev), int(system.has_constraints),
+                             L.stream_handle()), "fode_clamp")
+    return Y
+
+
+def project(system, Y, atol, rtol, counts: Counts, max_updates=10, tol=0.1,
+            max_backtracks=24, backtrack_factor=0.5, sigma=1.0e-4):
+    """Consistent initialization by damped Newton on the algebraic rows
+    (fastode/solvers.py:100-178), on device tensors.  Returns (Y, status);
+    a failure restores Y."""
+    alg_np = system.mass_np == 0.0
+    if not np.any(alg_np):
+        return Y, "converged"
+    dev = Y.device
+    alg = torch.as_tensor(alg_np, device=dev)
+    atol_a = as_device(atol)[alg]
+
+    def residual(Z):
+        G = system.rhs(Z)
+        counts.rhs += 1
+        return G * alg          # differential rows zeroed
+
+    def finite(A):
+        return bool(torch.isfinite(A[:, alg]).all())
+
+    def weighted(Z, D):
+        z, d = Z[:, alg], D[:, alg]
+        sc = atol_a[None, :] + rtol * torch.maximum(z.abs(), (z + d).abs())
+        q = torch.where(sc > 0.0, d.abs() / sc,
+                        torch.where(d == 0.0, torch.zeros_like(d), torch.full_like(d, math.inf)))
+        return float(q.max())
+
+    saved = None
+    for update in range(max_updates + 1):
+        Gv = residual(Y)
+        if not finite(Gv):
+            break
+        counts.proj_passes += 1
+        if float(Gv[:, alg].abs().max()) == 0.0:
+            return Y, "converged"
+        J = system.jac(Y)
+        counts.jac += 1
+        LU = system.factor(1.0, J, algebraic_rows_only=True)
+        counts.lu += 1
+        delta = LU.solve(Gv)
+        counts.solves += 1
+        if not finite(delta):
+            break
+        q = weighted(Y, delta)
+        if q <= tol:
+            return Y, "converged"
+        if update == max_updates:
+            break
+        if saved is None:
+            saved = Y.clone()
+        step = 1.0
+        taken = None
+        for _ in range(max_backtracks + 1):
+            cand = Y.clone()
+            cand[:, alg] += step * delta[:, alg]
+            used = step
+            step *= backtrack_factor
+            if not finite(cand):
+                continue
+            Gc = residual(cand)
+            counts.proj_trials += 1
+            if not finite(Gc):
+                continue
+            if float(Gc[:, alg].abs().max()) == 0.0:
+                taken = cand
+                break
+            trial = LU.solve(Gc)
+            counts.solves += 1
+            if not finite(trial):
+                continue
+            qt = weighted(cand, trial)
+            if qt <= tol or qt < (1.0 - sigma * used) * q:
+                taken = cand
+                break
+        if taken is None:
+            break
+        Y = taken
+    if saved is not None:
+        Y = saved
+    return Y, "projection failed"
+
+
+class Algorithm:
+    """The integrator of the reference's Section 4 (fastode/solvers.py:270-415)
+    on the device.  ``trace``, when set to a list, receives one (H, err,
+    accepted) tuple per attempt."""
+
+    name = "algorithm"
+
+    def __init__(self, tableau: Tableau = ROS3, controller: str = "h211b", b: float = 4.0,
+                 safety: float = 0.9, fac_min: float = 0.2, fac_max: float = 6.0,
+                 carry: bool = True, quantize: bool = True, max_steps: int = 1000,
+                 carry_memory: bool = True):
+        self.tableau = tableau
+        self.controller = controller
+        self.b = b
+        self.safety = safety
+        self.fac_min = fac_min
+        self.fac_max = fac_max
+        self.carry = carry
+        self.quantize = quantize
+        self.max_steps = max_steps
+        self.carry_memory = carry_memory
+        self.trace = None
+
+    def new_memory(self):
+        return Memory()
+
+    def solve(self, system, Y, dt, atol, rtol, counts: Counts, memory: Memory = None):
+        tab = self.tableau
+        mem = memory if memory is not None else Memory()
+        as_numpy = not isinstance(Y, torch.Tensor)
+        Yd = as_device(Y).clone()
+        C, n = Yd.shape
+        atol_d = as_device(atol)
+        constrained = system.has_constraints
+        params = ctrl_params(self, tab, constrained)
+        ctrl = GroupControl(float(dt), mem)
+        s = L.stream_handle()
+        L.check(L.lib.fode_ctrl_init(ct.byref(params), ctrl.ptr, 1, int(constrained), s), "ctrl_init")
+        host_counts = Counts()
+        if constrained:
+            Yp, st = project(system, Yd, atol, rtol, host_counts)
+            if st != "converged":
+                g = ctrl.read()
+                mem.err_prev, mem.h_prev = float(g["err_prev"]), float(g["h_prev"])
+                counts.add_array(g["counts"])
+                counts.add(host_counts)
+                return self._out(Yp, as_numpy), st
+            Yd = Yp.contiguous()
+            L.check(L.lib.fode_ctrl_begin(ct.byref(params), ctrl.ptr, 1, s), "ctrl_begin")
+        work = {k: torch.empty_like(Yd) for k in ("Yr", "rhs", "Ynew", "E")}
+        K = [torch.empty_like(Yd) for _ in range(tab.stages)]
+        g = ctrl.read()
+        F0 = J = None
+        while g["action"] != L.FINISH:
+            if g["action"] == L.NEW_STEP:
+                F0 = system.rhs(Yd)
+                J = system.jac(Yd)
+            H = float(g["H"])
+            LU = system.factor(ctrl.alpha(), J)
+            Ynew, E = stages(system, tab, params, Yd, F0, LU, ctrl, K, work)
+            L.check(L.lib.fode_norm_control(ct.byref(params), ctrl.ptr, C, n, L.ptr(Yd), L.ptr(Ynew),
+                                            L.ptr(E), L.ptr(atol_d), float(rtol),
+                                            L.ptr(system.mass_dev), s), "fode_norm_control")
+            g = ctrl.read()
+            if self.trace is not None:
+                self.trace.append((H, float(g["err"]), bool(g["accepted"])))
+        mem.h_next = float(g["h_next"])
+        mem.err_prev, mem.h_prev = float(g["err_prev"]), float(g["h_prev"])
+        counts.add_array(g["counts"])
+        counts.add(host_counts)
+        return self._out(Yd, as_numpy), L.STATUS[int(g["status"])]
+
+    @staticmethod
+    def _out(Y, as_numpy):
+        return Y.cpu().numpy() if as_numpy else Y
+
+
+def fixed_steps(system, tab, Y, t_end, n_steps, counts: Counts = None):
+    """n uncontrolled Rosenbrock steps of equal length (fastode/solvers.py:494-504)."""
+    alg = Algorithm(tab)
+    counts = counts or Counts()
+    Yd = as_device(Y).clone()
+    params = ctrl_params(alg, tab, system.has_constraints)
+    H = t_end / n_steps
+    ctrl = GroupControl(t_end, Memory())
+    ctrl.dev.view(F64)[1] = H                        # the H field read by the stage kernels
+    work = {k: torch.empty_like(Yd) for k in ("Yr", "rhs", "Ynew", "E")}
+    K = [torch.empty_like(Yd) for _ in range(tab.stages)]
+    for _ in range(n_steps):
+        F0 = system.rhs(Yd)
+        J = system.jac(Yd)
+        LU = system.factor(1.0 / (H * tab.gamma), J)
+        Ynew, _ = stages(system, tab, params, Yd, F0, LU, ctrl, K, work)
+        Yd = Ynew.clone()
+    return Yd

@@ -1,0 +1,159 @@
+/* C ABI of the MI355X (gfx950) batched Rosenbrock integrator.
+ *
+ * Every entry point takes caller-owned device pointers (FP64 unless noted)
+ * and a hipStream_t passed as void*; the library never frees caller memory.
+ * Return value: 0 on success, a negative code on a bad argument or a HIP
+ * error (fode_last_error() gives the text).  Numerical failures are not
+ * errors: they come back as per-group status codes, as the reference returns
+ * status strings (fastode/solvers.py:330-367).
+ *
+ * Layouts.  A group's state Y is (C, n) row-major (cell-major: one cell's
+ * species contiguous), rate constants k are (C, R), Jacobian values are
+ * (C, nnz) in the mechanism's pattern order, a stored factorization is
+ * (C, lu_len).  One device workgroup owns one cell of the LU.
+ *
+ * Handles are not thread-safe; calls on one stream are ordered.
+ */
+#ifndef FASTODE_HIP_H
+#define FASTODE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes, mapped to the reference's strings by the host code. */
+enum {
+  FODE_RUNNING = 0,
+  FODE_CONVERGED = 1,        /* "converged" */
+  FODE_TOO_MANY_STEPS = 2,   /* "too many steps" */
+  FODE_STEP_TOO_SMALL = 3,   /* "step too small" */
+  FODE_NAN_OR_INF = 4,       /* "nan or inf" */
+  FODE_PROJECTION_FAILED = 5 /* "projection failed" */
+};
+
+/* Next action of a group after a control update. */
+enum { FODE_FINISH = 0, FODE_NEW_STEP = 1, FODE_RETRY = 2 };
+
+/* Operation counters, 13 x int64 per group, in the order of
+ * fastode/counts.py:9-23. */
+enum {
+  FODE_C_STEPS = 0, FODE_C_ACCEPTED, FODE_C_REJECTED, FODE_C_RHS, FODE_C_JAC, FODE_C_LU,
+  FODE_C_SOLVES, FODE_C_NEWTON, FODE_C_FAILURES, FODE_C_UNCONVERGED, FODE_C_PROJ_PASSES,
+  FODE_C_PROJ_TRIALS, FODE_C_CALLS, FODE_NCOUNTS
+};
+
+/* Per-group controller state on the device: the local variables of
+ * Algorithm.solve (fastode/solvers.py:305-415) plus the carried Memory
+ * (fastode/solvers.py:181-186) and the group's Counts. */
+typedef struct fode_group_ctrl {
+  double t, H, H_ctrl, proposed, dt, h_min, h_max, err, alpha;
+  double h_next, err_prev, h_prev; /* Memory */
+  int32_t nsteps, status, last, shortened, rej_last, rej_more, action, accepted;
+  int64_t counts[FODE_NCOUNTS];
+} fode_group_ctrl;
+
+/* Algorithm options (fastode/solvers.py:275-292) and tableau constants. */
+typedef struct fode_ctrl_params {
+  double b, safety, fac_min, fac_max, k_exp, gamma;
+  int32_t h211b, carry, quantize, carry_memory, max_steps, stages, n_new_forcing, constrained;
+  double a[16], c[16], m[4], e[4];
+  int32_t new_forcing[4];
+} fode_ctrl_params;
+
+/* Mechanism tables (see the package's mechanism.py). */
+typedef struct fode_mech_desc {
+  int32_t n, R, P, nnz, ncon;
+  const int32_t* react;      /* R*P, pad = n */
+  const int32_t* rhs_ptr;    /* n+1 */
+  const int32_t* rhs_rxn;
+  const double* rhs_coef;
+  const int32_t* jac_ptr;    /* nnz+1 */
+  const int32_t* jac_src;    /* j*P + p */
+  const double* jac_coef;
+  const double* jac_const;   /* nnz */
+  const int32_t* con_rows;   /* ncon */
+  const double* con_coef;    /* ncon*n */
+  const double* con_total;   /* ncon */
+} fode_mech_desc;
+
+/* Symbolic LU plan (see the package's symbolic.py). */
+typedef struct fode_plan_desc {
+  int32_t n, c, m, tile, ns, nnz;
+  int32_t f_ntasks, f_nst, f_npairs;
+  int32_t schur_rows;
+  int32_t sf_ntasks, sf_nst, sb_ntasks, sb_nst, s_npairs;
+  const int32_t* sp_src; const int32_t* sp_diag; const int32_t* sp_row;     /* ns */
+  const int32_t* core_src; const int32_t* core_diag; const int32_t* core_row; /* tile^2*256 */
+  const int32_t* f_tasks; const int32_t* f_ptr; const uint32_t* f_pairs;
+  const int32_t* schur_ptr; const uint32_t* schur;
+  const int32_t* sf_tasks; const int32_t* sf_ptr;
+  const int32_t* sb_tasks; const int32_t* sb_ptr;
+  const uint32_t* s_pairs;
+  const int32_t* perm;       /* n: new -> old */
+} fode_plan_desc;
+
+const char* fode_last_error(void);
+
+/* Device properties the host needs (LDS per workgroup, CUs). */
+int fode_device_info(int device, int* lds_per_block, int* cus, int* warp_size);
+
+/* Mechanism: replaces Mechanism's compiled tables (fastode/chem.py:19-79). */
+int fode_mech_create(const fode_mech_desc* desc, void** out);
+int fode_mech_destroy(void* mech);
+
+/* Plan: replaces SparsePattern + SuperLU's per-call symbolic work
+ * (fastode/linalg.py:35-81). lu_len receives doubles per cell of a factor. */
+int fode_plan_create(const fode_plan_desc* desc, void** out, int64_t* lu_len);
+int fode_plan_destroy(void* plan);
+
+/* F = rhs(Y): Mechanism.rhs (fastode/chem.py:86-90) with linear constraint
+ * rows. Y (C,n), k (C,R), F (C,n). */
+int fode_rhs(void* mech, int C, const double* Y, const double* k, double* F, void* stream);
+
+/* J = jac(Y): Mechanism.jac_values (fastode/chem.py:92-102). J (C,nnz). */
+int fode_jac(void* mech, int C, const double* Y, const double* k, double* J, void* stream);
+
+/* LU of alpha*diag(mass) - J (or diag(mass) - J restricted to algebraic rows
+ * when alg_only), per cell: SparsePattern.matrix_values + SparseLU /
+ * dense_matrix + DenseLU (fastode/linalg.py:15-32, 60-81). alpha is a device
+ * scalar. J holds nnz values per cell in the plan's pattern order. */
+int fode_factor(void* plan, int C, const double* alpha, const double* mass, const double* J,
+                int alg_only, double* LU, void* stream);
+
+/* X = LU.solve(B) (fastode/linalg.py:21, 80-81). B, X (C,n); may alias. */
+int fode_solve(void* plan, int C, const double* LU, const double* B, double* X, void* stream);
+
+/* Controller, one group: ctrl_init = the set-up of Algorithm.solve
+ * (fastode/solvers.py:306-320) and, unless begin_later, the first loop-top
+ * checks; ctrl_begin = those checks (after a projection). */
+int fode_ctrl_init(const fode_ctrl_params* p, fode_group_ctrl* g, int ngroups, int begin_later,
+                   void* stream);
+int fode_ctrl_begin(const fode_ctrl_params* p, fode_group_ctrl* g, int ngroups, void* stream);
+
+/* Stage arithmetic of `stages` (fastode/solvers.py:54-86), group of C cells.
+ * K holds the stage vectors, each (C,n). */
+int fode_stage_point(const fode_ctrl_params* p, int r, int C, int n, const double* Y,
+                     const double* const* K, double* out, void* stream);
+int fode_stage_forcing(const fode_ctrl_params* p, int r, int C, int n, const fode_group_ctrl* g,
+                       const double* F, const double* const* K, const double* mass, int dae,
+                       double* out, void* stream);
+int fode_stage_final(const fode_ctrl_params* p, int C, int n, const double* Y,
+                     const double* const* K, double* Ynew, double* E, void* stream);
+
+/* error_norm (fastode/solvers.py:37-43) over the group, then the
+ * accept/reject update of Algorithm.solve (fastode/solvers.py:362-411) on
+ * the device; on acceptance Y <- Ynew, on finishing Y is clamped
+ * (fastode/solvers.py:89-97, 412-415). */
+int fode_norm_control(const fode_ctrl_params* p, fode_group_ctrl* g, int C, int n, double* Y,
+                      const double* Ynew, const double* E, const double* atol, double rtol,
+                      const double* mass, void* stream);
+
+/* clamp(system, Y) alone (fastode/solvers.py:89-97). */
+int fode_clamp(int C, int n, double* Y, const double* mass, int constrained, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

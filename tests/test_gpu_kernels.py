@@ -1,0 +1,160 @@
+"""GPU: the HIP kernels against the oracle and NumPy, through the C ABI."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import fastode_oracle as O
+from fastode_amd import workloads as W
+from fastode_amd.mechanism import synthetic
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(x):
+    return torch.as_tensor(x, dtype=torch.float64, device="cuda").contiguous()
+
+
+def oracle_mech(mech):
+    cons = {mech.species[r]: ({mech.species[s]: mech.constraint_coef[i, s]
+                               for s in np.nonzero(mech.constraint_coef[i])[0]}, mech.constraint_total[i])
+            for i, r in enumerate(mech.constraint_rows)} or None
+    return O.OracleMechanism(mech.species, mech.reactions_list(), cons)
+
+
+def test_device_info():
+    import fastode_amd._lib as L
+    info = L.device_info(0)
+    print("device:", info, torch.cuda.get_device_name(0))
+    assert info["warp_size"] == 64 and info["cus"] > 0 and info["lds_per_block"] >= 65536
+
+
+@pytest.mark.parametrize("which", ["small", "diurnal200", "constrained100"])
+def test_rhs_and_jacobian_match_oracle(which):
+    from fastode_amd.system import ChemSystem
+    wl = {"small": lambda: W.small(C=64), "diurnal200": lambda: W.diurnal200(C=64),
+          "constrained100": lambda: W.constrained100(C=64)}[which]()
+    om = oracle_mech(wl.mech)
+    k = wl.k(0)
+    system = ChemSystem(wl.mech, k, mass=wl.mass)
+    Y = wl.Y0
+    F = system.rhs(dev(Y)).cpu().numpy()
+    Fo = om.rhs(k, Y)
+    scale = np.abs(Fo).max(axis=1, keepdims=True)
+    assert np.abs(F - Fo).max() <= 1e-13 * scale.max()
+    assert (np.abs(F - Fo) <= 1e-12 * scale + 1e-300).all()
+    J = system.jac(dev(Y)).cpu().numpy()
+    Jo = om.jac_values(k, Y)
+    assert J.shape == Jo.shape
+    assert np.allclose(J, Jo, rtol=1e-13, atol=0.0)
+
+
+@pytest.mark.parametrize("n,R,seed,alpha", [(20, 40, 1, 1e-2), (100, 250, 4, 1e-3),
+                                              (200, 500, 2, 1e-3), (200, 500, 2, 10.0)])
+def test_factor_and_solve_match_numpy(n, R, seed, alpha):
+    """alpha*I - J of a mass-action Jacobian at a log-uniform state, factored
+    without pivoting on the device, solved and compared with LAPACK."""
+    from fastode_amd.mechanism import log_uniform_state
+    from fastode_amd.system import ChemSystem
+    syn = synthetic(n, R, seed)
+    mech = syn.mech
+    rng = np.random.default_rng(seed)
+    C = 8
+    Y = log_uniform_state(C, n, rng)
+    k = np.broadcast_to(syn.k, (C, R)).copy()
+    system = ChemSystem(mech, k)
+    J = system.jac(dev(Y))
+    LU = system.factor(alpha, J)
+    B = rng.standard_normal((C, n))
+    X = LU.solve(dev(B)).cpu().numpy()
+    om = oracle_mech(mech)
+    Jd = om.dense(J.cpu().numpy())
+    for c in range(C):
+        A = alpha * np.eye(n) - Jd[c]
+        xr = np.linalg.solve(A, B[c])
+        res = np.abs(A @ X[c] - B[c]).max() / (np.abs(A).max() * np.abs(X[c]).max())
+        assert res < 1e-13, (c, res)
+        assert np.abs(X[c] - xr).max() <= 1e-8 * np.abs(xr).max()
+
+
+@pytest.mark.parametrize("n", [1, 3, 17, 40])
+def test_dense_system_factor(n):
+    from fastode_amd.system import DenseSystem
+    rng = np.random.default_rng(n)
+    C = 5
+    Jm = rng.standard_normal((C, n, n))
+    system = DenseSystem(n, lambda Y: Y, lambda Y: dev(Jm))
+    alpha = 3.0 * n + 1.0
+    LU = system.factor(alpha, dev(Jm))
+    B = rng.standard_normal((C, n))
+    X = LU.solve(dev(B)).cpu().numpy()
+    for c in range(C):
+        xr = np.linalg.solve(alpha * np.eye(n) - Jm[c], B[c])
+        assert np.allclose(X[c], xr, rtol=1e-12, atol=1e-14)
+
+
+def test_algebraic_rows_only_factor():
+    from fastode_amd.system import DenseSystem
+    n, C = 4, 3
+    rng = np.random.default_rng(5)
+    Jm = rng.standard_normal((C, n, n))
+    mass = np.array([1.0, 0.0, 1.0, 0.0])
+    system = DenseSystem(n, lambda Y: Y, lambda Y: dev(Jm), mass=mass)
+    LU = system.factor(1.0, dev(Jm), algebraic_rows_only=True)
+    B = rng.standard_normal((C, n))
+    X = LU.solve(dev(B)).cpu().numpy()
+    A = O.stage_matrix_dense(1.0, mass, Jm, algebraic_rows_only=True)
+    for c in range(C):
+        assert np.allclose(X[c], np.linalg.solve(A[c], B[c]), rtol=1e-12, atol=1e-14)
+
+
+def test_stage_kernels_match_the_reference_formulas():
+    import ctypes as ct
+    import fastode_amd._lib as L
+    from fastode_amd import Algorithm, Memory, ROS3, RODAS3
+    from fastode_amd.solvers import GroupControl, ctrl_params
+    rng = np.random.default_rng(3)
+    C, n = 32, 7
+    for tab in (ROS3, RODAS3):
+        p = ctrl_params(Algorithm(tab), tab, False)
+        H = 0.37
+        ctrl = GroupControl(1.0, Memory())
+        ctrl.dev.view(torch.float64)[1] = H
+        Y = rng.standard_normal((C, n))
+        F = rng.standard_normal((C, n))
+        K = [rng.standard_normal((C, n)) for _ in range(tab.stages)]
+        Kd = [dev(x) for x in K]
+        mass = np.array([1, 1, 0, 1, 1, 1, 0], dtype=float)
+        out = torch.empty((C, n), dtype=torch.float64, device="cuda")
+        s = L.stream_handle()
+        for r in range(1, tab.stages):
+            L.check(L.lib.fode_stage_point(ct.byref(p), r, C, n, L.ptr(dev(Y)), L.ptr_array(Kd[:r]),
+                                           L.ptr(out), s))
+            want = Y.copy()
+            for q in range(r):
+                if tab.a[r, q] != 0.0:
+                    want += tab.a[r, q] * K[q]
+            assert np.array_equal(out.cpu().numpy(), want)
+            for dae in (0, 1):
+                L.check(L.lib.fode_stage_forcing(ct.byref(p), r, C, n, ctrl.ptr, L.ptr(dev(F)),
+                                                 L.ptr_array(Kd[:r]), L.ptr(dev(mass)), dae,
+                                                 L.ptr(out), s))
+                want = F.copy()
+                for q in range(r):
+                    coef = tab.c[r, q] / H
+                    if coef != 0.0:
+                        want += coef * (K[q] * mass[None, :] if dae else K[q])
+                assert np.array_equal(out.cpu().numpy(), want)
+        Ynew = torch.empty_like(out)
+        E = torch.empty_like(out)
+        L.check(L.lib.fode_stage_final(ct.byref(p), C, n, L.ptr(dev(Y)), L.ptr_array(Kd), L.ptr(Ynew),
+                                       L.ptr(E), s))
+        wy, we = Y.copy(), np.zeros_like(Y)
+        for r in range(tab.stages):
+            if tab.m[r] != 0.0:
+                wy += tab.m[r] * K[r]
+            if tab.e[r] != 0.0:
+                we += tab.e[r] * K[r]
+        assert np.array_equal(Ynew.cpu().numpy(), wy) and np.array_equal(E.cpu().numpy(), we)
+        _ = math

@@ -1,0 +1,133 @@
+"""CPU checks of the host side: the symbolic LU plan (through a CPU model of
+exactly what the kernels do with it), the mechanism tables against the oracle,
+and the C ABI library's exported symbols."""
+import ctypes as ct
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import fastode_oracle as O
+from fastode_amd import workloads as W
+from fastode_amd.mechanism import Mechanism, synthetic
+from fastode_amd.symbolic import G, build_plan, emulate_factor, emulate_solve
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def random_matrix(n, rows, cols, rng, dominance=20.0):
+    A = np.zeros((n, n))
+    A[rows, cols] = rng.standard_normal(len(rows))
+    A[np.arange(n), np.arange(n)] += dominance
+    return A
+
+
+@pytest.mark.parametrize("n,R,seed", [(20, 40, 1), (100, 250, 4), (200, 500, 2), (200, 500, 7)])
+def test_plan_factor_and_solve_match_numpy(n, R, seed):
+    mech = synthetic(n, R, seed).mech
+    plan = build_plan(n, mech.rows, mech.cols)
+    assert plan.c + plan.m == n and plan.m <= plan.m_pad
+    rng = np.random.default_rng(seed)
+    A = random_matrix(n, mech.rows, mech.cols, rng)
+    lds, core = emulate_factor(plan, A)
+    for _ in range(2):
+        b = rng.standard_normal(n)
+        x = emulate_solve(plan, lds, core, b)
+        xr = np.linalg.solve(A, b)
+        assert np.abs(x - xr).max() <= 1e-12 * np.abs(xr).max()
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 16, 17, 40])
+def test_dense_plans(n):
+    rows = np.repeat(np.arange(n), n)
+    cols = np.tile(np.arange(n), n)
+    plan = build_plan(n, rows, cols, order=np.arange(n))
+    assert plan.c == 0 and plan.ns == 0 and plan.tile == -(-n // G)
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n)) + 3 * n * np.eye(n)
+    lds, core = emulate_factor(plan, A)
+    b = rng.standard_normal(n)
+    assert np.allclose(emulate_solve(plan, lds, core, b), np.linalg.solve(A, b), rtol=1e-12, atol=1e-14)
+
+
+def test_plan_fill_is_a_superset_and_levels_are_causal():
+    mech = synthetic(200, 500, 2).mech
+    plan = build_plan(mech.n, mech.rows, mech.cols)
+    inv = np.empty(mech.n, dtype=np.int64)
+    inv[plan.perm] = np.arange(mech.n)
+    assert plan.F[inv[mech.rows], inv[mech.cols]].all()
+    # every factor task's pairs read entries final before its stage
+    a = plan.arrays
+    tasks = a["f_tasks"].reshape(-1, 4)
+    ptr = a["f_ptr"]
+    done_stage = {}
+    for s in range(len(ptr) - 1):
+        for t in range(ptr[s], ptr[s + 1]):
+            done_stage[int(tasks[t, 0])] = s
+    for s in range(len(ptr) - 1):
+        for t in range(ptr[s], ptr[s + 1]):
+            for q in range(tasks[t, 1], tasks[t, 1] + tasks[t, 2]):
+                w = int(a["f_pairs"][q])
+                for src in (w >> 16, w & 0xFFFF):
+                    assert done_stage.get(src, -1) < s
+
+
+def test_mechanism_tables_match_the_oracle():
+    wl = W.small()
+    mech = wl.mech
+    om = O.OracleMechanism(mech.species, mech.reactions_list())
+    rng = np.random.default_rng(0)
+    Y = wl.Y0[:4]
+    k = wl.k(0)[:4]
+    # forcing through the gather tables
+    rates = k * np.prod(np.concatenate([Y, np.ones((4, 1))], axis=1)[:, mech.react], axis=2)
+    F = np.zeros_like(Y)
+    for i in range(mech.n):
+        for q in range(mech.rhs_ptr[i], mech.rhs_ptr[i + 1]):
+            F[:, i] += rates[:, mech.rhs_rxn[q]] * mech.rhs_coef[q]
+    assert np.allclose(F, om.rhs(k, Y), rtol=1e-13, atol=1e-30)
+    # Jacobian values through the gather tables, same pattern order
+    assert np.array_equal(mech.rows, om.rows) and np.array_equal(mech.cols, om.cols)
+    Jo = om.jac_values(k, Y)
+    X = np.concatenate([Y, np.ones((4, 1))], axis=1)[:, mech.react]
+    d = np.stack([k * X[:, :, 1 - p] for p in range(2)], axis=2).reshape(4, -1)
+    J = np.zeros_like(Jo)
+    for e in range(mech.nnz):
+        for q in range(mech.jac_ptr[e], mech.jac_ptr[e + 1]):
+            J[:, e] += mech.jac_coef[q] * d[:, mech.jac_src[q]]
+    assert np.allclose(J, Jo, rtol=1e-13, atol=1e-30)
+    _ = rng
+
+
+def test_constrained_mechanism_rows():
+    wl = W.constrained100(C=8)
+    mech = wl.mech
+    assert len(mech.constraint_rows) == 3
+    # the initial state satisfies each conservation row
+    g = wl.Y0 @ mech.constraint_coef.T - mech.constraint_total[None, :]
+    assert np.abs(g).max() < 1e-15
+    assert (wl.mass[mech.constraint_rows] == 0).all() and wl.mass.sum() == 97
+
+
+def test_reactions_roundtrip():
+    mech = synthetic(30, 60, 3).mech
+    again = Mechanism(mech.species, mech.reactions_list())
+    assert np.array_equal(again.react, mech.react) and np.array_equal(again.S, mech.S)
+
+
+def test_abi_library_exports_every_declared_symbol():
+    header = (ROOT / "include" / "fastode_hip.h").read_text()
+    declared = set(re.findall(r"\b(fode_[a-z_]+)\s*\(", header))
+    import fastode_amd._lib as L
+    for name in declared:
+        assert hasattr(L.lib, name), name
+        assert isinstance(getattr(L.lib, name), ct._CFuncPtr)
+    assert declared == set(L.EXPORTS)
+
+
+def test_ctrl_record_layout_matches_header():
+    import fastode_amd._lib as L
+    # 12 doubles, 8 int32, 13 int64 (include/fastode_hip.h: fode_group_ctrl)
+    assert L.CTRL_DTYPE.itemsize == 12 * 8 + 8 * 4 + 13 * 8
+    assert ct.sizeof(L.CtrlParams) == 6 * 8 + 8 * 4 + 16 * 8 * 2 + 4 * 8 * 2 + 4 * 4
