@@ -32,9 +32,10 @@ CTRL_DTYPE = np.dtype([
     ("h_next", "f8"), ("err_prev", "f8"), ("h_prev", "f8"),
     ("nsteps", "i4"), ("status", "i4"), ("last", "i4"), ("shortened", "i4"),
     ("rej_last", "i4"), ("rej_more", "i4"), ("action", "i4"), ("accepted", "i4"),
+    ("fails", "i4"), ("first_fail", "i4"),
     ("counts", "i8", (NCOUNTS,)),
 ])
-assert CTRL_DTYPE.itemsize == 232
+assert CTRL_DTYPE.itemsize == 240
 CTRL_WORDS = CTRL_DTYPE.itemsize // 8
 ALPHA_OFFSET = CTRL_DTYPE.fields["alpha"][1]
 
@@ -74,6 +75,14 @@ class PlanDesc(ct.Structure):
                 ("s_pairs", ct.c_void_p), ("perm", ct.c_void_p)]
 
 
+class BatchDesc(ct.Structure):
+    _fields_ = [("C", ct.c_int32), ("group_size", ct.c_int32), ("ngroups", ct.c_int32),
+                ("k_stride", ct.c_int32), ("dt", ct.c_double), ("rtol", ct.c_double),
+                ("Y", ct.c_void_p), ("k", ct.c_void_p), ("kmul", ct.c_void_p), ("atol", ct.c_void_p),
+                ("mass", ct.c_void_p), ("g", ct.c_void_p), ("work", ct.c_void_p),
+                ("queue", ct.c_void_p)]
+
+
 vp = ct.c_void_p
 i32 = ct.c_int32
 P = ct.POINTER
@@ -96,6 +105,11 @@ _SIGS = {
     "fode_stage_final": (i32, [P(CtrlParams), i32, i32, vp, P(vp), vp, vp, vp]),
     "fode_norm_control": (i32, [P(CtrlParams), vp, i32, i32, vp, vp, vp, vp, ct.c_double, vp, vp]),
     "fode_clamp": (i32, [i32, i32, vp, vp, i32, vp]),
+    "fode_batched_interval": (i32, [vp, vp, P(CtrlParams), P(BatchDesc), i32, vp]),
+    "fode_batched_lds_bytes": (i32, [vp, vp, i32, P(ct.c_int64)]),
+    "fode_batched_graph_create": (i32, [vp, vp, P(CtrlParams), P(BatchDesc), i32, P(vp)]),
+    "fode_graph_launch": (i32, [vp, vp]),
+    "fode_graph_destroy": (i32, [vp]),
 }
 
 for _name, (_res, _args) in _SIGS.items():

@@ -1,0 +1,148 @@
+"""Batched persistent form: every group of a batch over one host interval in
+one launch (``fode_batched_interval``), optionally replayed from a hipGraph.
+
+It replaces the reference's group loop around ``Algorithm.solve``
+(fastode/ts1.py:196-216, group-outer and interval-inner) by interval-outer,
+group-parallel launches with the same per-group semantics: each group of
+``group_size`` consecutive cells keeps its own controller state and Memory
+on the device across calls, and its steps, Counts and status are those a
+separate ``Algorithm.solve`` call per group would give.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from .counts import COUNT_FIELDS, Counts, Memory
+from .device import device_mechanism, mechanism_plan
+from .solvers import Algorithm, ctrl_params
+from .system import F64, as_device, device
+from .tableaus import ROS3
+
+
+class GroupMemories:
+    """The device controller blocks of all groups (fode_group_ctrl each):
+    the carried Memory, the running Counts and the failure record."""
+
+    def __init__(self, ngroups):
+        rec = np.zeros(ngroups, dtype=L.CTRL_DTYPE)
+        self.dev = torch.from_numpy(rec.view(np.int64).copy()).to(device())
+        self.ngroups = ngroups
+
+    def read(self):
+        return self.dev.cpu().numpy().view(L.CTRL_DTYPE)
+
+    def memory(self, group) -> Memory:
+        r = self.read()[group]
+        return Memory(float(r["h_next"]), float(r["err_prev"]), float(r["h_prev"]))
+
+    def counts(self) -> np.ndarray:
+        return self.read()["counts"].copy()
+
+
+class BatchedAlgorithm:
+    """``Algorithm`` semantics (same options) over a whole batch of groups."""
+
+    name = "batched-algorithm"
+
+    def __init__(self, tableau=ROS3, group_size=32, blocks_per_cu=0, **options):
+        self.alg = Algorithm(tableau, **options)
+        self.tableau = tableau
+        self.group_size = group_size
+        self.blocks_per_cu = blocks_per_cu
+
+    def new_memory(self, C):
+        return GroupMemories(-(-C // self.group_size))
+
+    def runner(self, system, Y, dt, atol, rtol, memory=None, graph=True):
+        return BatchedRunner(self, system, Y, dt, atol, rtol, memory, graph)
+
+    def solve(self, system, Y, dt, atol, rtol, counts: Counts, memory=None, kmul=None):
+        """One host interval for every group. Returns (Y_new, statuses), the
+        statuses a list of per-group strings."""
+        r = BatchedRunner(self, system, Y, dt, atol, rtol, memory, graph=False)
+        before = r.memory.counts()
+        r.step(kmul)
+        rec = r.memory.read()
+        d = rec["counts"] - before
+        for name, v in zip(COUNT_FIELDS, d.sum(axis=0)):
+            setattr(counts, name, getattr(counts, name) + int(v))
+        statuses = [L.STATUS[int(s)] for s in rec["status"]]
+        out = r.Y.clone()
+        return (out.cpu().numpy() if not isinstance(Y, torch.Tensor) else out), statuses
+
+
+class BatchedRunner:
+    """Device buffers and (optionally) the captured interval graph of one
+    batch; ``step(kmul)`` advances every group by one interval."""
+
+    def __init__(self, balg: BatchedAlgorithm, system, Y, dt, atol, rtol, memory=None, graph=True):
+        if system.has_constraints and balg.tableau.name != "rodas3":
+            raise ValueError("constrained systems need RODAS3")
+        if system.has_constraints:
+            raise NotImplementedError("the batched form does not project constrained systems yet")
+        self.balg = balg
+        self.system = system
+        self.Y = as_device(Y).clone()
+        C, n = self.Y.shape
+        self.C, self.n = C, n
+        gs = balg.group_size
+        self.ngroups = -(-C // gs)
+        self.memory = memory if memory is not None else balg.new_memory(C)
+        if self.memory.ngroups != self.ngroups:
+            raise ValueError("memory holds a different number of groups")
+        k = system.k
+        if k.shape[0] not in (1, C):
+            raise ValueError("k must have 1 or C rows")
+        self.k = k
+        self.kmul = torch.ones(system.mech.R, dtype=F64, device=self.Y.device)
+        self.atol = as_device(atol)
+        self.work = torch.empty_like(self.Y)
+        self.queue = torch.zeros(1, dtype=torch.int32, device=self.Y.device)
+        self.params = ctrl_params(balg.alg, balg.tableau, system.has_constraints)
+        self.plan = mechanism_plan(system.mech)
+        self.dmech = device_mechanism(system.mech)
+        self.desc = L.BatchDesc(
+            C=C, group_size=gs, ngroups=self.ngroups, k_stride=(0 if k.shape[0] == 1 else k.shape[1]),
+            dt=float(dt), rtol=float(rtol), Y=self.Y.data_ptr(), k=k.data_ptr(),
+            kmul=self.kmul.data_ptr(), atol=self.atol.data_ptr(), mass=system.mass_dev.data_ptr(),
+            g=self.memory.dev.data_ptr(), work=self.work.data_ptr(), queue=self.queue.data_ptr())
+        self.graph = None
+        if graph:
+            h = ct.c_void_p()
+            L.check(L.lib.fode_batched_graph_create(self.plan.handle, self.dmech.handle,
+                                                    ct.byref(self.params), ct.byref(self.desc),
+                                                    int(balg.blocks_per_cu), ct.byref(h)),
+                    "fode_batched_graph_create")
+            self.graph = h
+
+    def set_kmul(self, kmul):
+        if kmul is None:
+            self.kmul.fill_(1.0)
+        else:
+            self.kmul.copy_(torch.as_tensor(kmul, dtype=F64), non_blocking=False)
+
+    def step(self, kmul=None):
+        self.set_kmul(kmul)
+        s = L.stream_handle()
+        if self.graph is not None:
+            L.check(L.lib.fode_graph_launch(self.graph, s), "fode_graph_launch")
+        else:
+            L.check(L.lib.fode_batched_interval(self.plan.handle, self.dmech.handle, ct.byref(self.params),
+                                                ct.byref(self.desc), int(self.balg.blocks_per_cu), s),
+                    "fode_batched_interval")
+
+    def lds_bytes(self):
+        b = ct.c_int64()
+        L.check(L.lib.fode_batched_lds_bytes(self.plan.handle, self.dmech.handle, self.balg.tableau.stages,
+                                             ct.byref(b)), "fode_batched_lds_bytes")
+        return b.value
+
+    def __del__(self):
+        g = getattr(self, "graph", None)
+        if g:
+            L.lib.fode_graph_destroy(g)
+            self.graph = None

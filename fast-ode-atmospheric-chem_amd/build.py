@@ -8,7 +8,7 @@ import sys
 from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
-SRC = HERE / "csrc" / "fastode_hip.hip"
+SRCS = [HERE / "csrc" / "fastode_hip.hip", HERE / "csrc" / "batched.hip"]
 LIB = HERE / "libfastode_hip.so"
 ARCH = os.environ.get("FODE_OFFLOAD_ARCH", "gfx950")
 
@@ -26,7 +26,7 @@ FLAGS = [
 def build(verbose=False) -> Path:
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc, *FLAGS, "-I", str(HERE.parent / "include"), str(SRC), "-o", str(tmp)]
+    cmd = [hipcc, *FLAGS, "-I", str(HERE.parent / "include"), *map(str, SRCS), "-o", str(tmp)]
     if verbose:
         print(" ".join(cmd), file=sys.stderr)
     subprocess.run(cmd, check=True)

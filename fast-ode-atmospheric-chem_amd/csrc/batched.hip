@@ -1,0 +1,349 @@
+// Batched persistent form: one launch integrates every group of a batch over
+// one host interval.
+//
+// Workgroups (256 threads) are persistent and take groups from an atomic
+// queue.  A workgroup runs its group's whole step loop (the controller of
+// Algorithm.solve, fastode/solvers.py:305-415, on the device) and performs
+// every attempt cell by cell: the cell's rates, Jacobian, stage matrix and
+// its LU, the stage solves and the stage combination all stay in LDS and
+// registers; only the cell's state, rate constants and the new state touch
+// memory.  The group's error is the sum of the cells' partial sums in cell
+// order, exactly as the plain form forms it, so both forms take the same
+// steps.
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "engine.hpp"
+
+using namespace fode;
+
+namespace {
+
+}  // namespace
+
+// Shared with fastode_hip.hip, which owns the handles and the error text.
+extern "C" const void* fode_internal_plan(void* plan);
+extern "C" const void* fode_internal_mech(void* mech);
+extern "C" void fode_internal_set_error(const char* msg);
+
+namespace {
+
+int bfail(const std::string& m) {
+  fode_internal_set_error(m.c_str());
+  return -1;
+}
+
+struct BatchArgs {
+  PlanDev P;
+  MechDev M;
+  fode_ctrl_params p;
+  fode_batch_desc b;
+  int vec_off;    // LDS offsets (doubles) of the cell region
+};
+
+// LDS of the cell region, after the engine region.
+struct CellLds {
+  double* y;      // n
+  double* f0;     // n   F(Y)
+  double* fr;     // n   forcing of a later stage
+  double* w;      // n   stage point / new state
+  double* K;      // S*n
+  double* kc;     // R   rate constants
+  double* rd;     // max(R*P, R): rate partials, rates (aliases the engine's scr)
+  double* red;    // 4
+  int* flags;     // 4 ints
+};
+
+__host__ __device__ inline int cell_region_doubles(int n, int R, int S) {
+  return 4 * n + S * n + R + 4 + 2;
+}
+
+template <int TILE>
+__device__ inline void rates_into(const MechDev& M, const double* y, const double* kc, double* rates) {
+  for (int j = threadIdx.x; j < M.R; j += NT) rates[j] = rate(M, y, kc, j);
+  __syncthreads();
+}
+
+__device__ inline void forcing_into(const MechDev& M, const double* y, const double* rates, double* out) {
+  for (int i = threadIdx.x; i < M.n; i += NT) out[i] = forcing(M, y, rates, i);
+  __syncthreads();
+}
+
+// One attempt on one cell. Returns the cell's error partial sum (thread 0).
+template <int TILE>
+__device__ double cell_attempt(const BatchArgs& A, const EngineLds& L, const CellLds& V,
+                               const fode_group_ctrl& g, int cell, double* ynew_out) {
+  const PlanDev& P = A.P;
+  const MechDev& M = A.M;
+  const fode_ctrl_params& p = A.p;
+  const int n = M.n, tid = threadIdx.x;
+  const double* yg = A.b.Y + (size_t)cell * n;
+  const double* kg = A.b.k + (size_t)cell * A.b.k_stride;
+  for (int i = tid; i < n; i += NT) V.y[i] = yg[i];
+  for (int j = tid; j < M.R; j += NT) V.kc[j] = kg[j] * A.b.kmul[j];
+  __syncthreads();
+  // F(Y) (fastode/solvers.py:345) and the Jacobian's rate partials (:347).
+  rates_into<TILE>(M, V.y, V.kc, V.rd);
+  forcing_into(M, V.y, V.rd, V.f0);
+  for (int jp = tid; jp < M.R * M.P; jp += NT) V.rd[jp] = drate(M, V.y, V.kc, jp / M.P, jp % M.P);
+  __syncthreads();
+  double a[TILE][TILE], dLt[TILE], dUt[TILE];
+  assemble<TILE>(P, JFromD{M, V.rd}, g.alpha, A.b.mass, false, L.lds, a);
+  factor_cell<TILE>(P, L, a, dLt, dUt);
+  // Stages (fastode/solvers.py:54-86).
+  double* vec = L.lds + P.ns + 1;
+  const int S = p.stages;
+  const bool dae = p.constrained != 0;
+  const double* forcing_src = V.f0;
+  for (int r = 0; r < S; ++r) {
+    if (r > 0 && p.new_forcing[r]) {
+      for (int i = tid; i < n; i += NT) {
+        double v = V.y[i];
+        for (int s = 0; s < r; ++s) {
+          const double as = p.a[r * 4 + s];
+          if (as != 0.0) v = v + as * V.K[s * n + i];
+        }
+        V.w[i] = v;
+      }
+      __syncthreads();
+      rates_into<TILE>(M, V.w, V.kc, V.rd);
+      forcing_into(M, V.w, V.rd, V.fr);
+      forcing_src = V.fr;
+    }
+    // right-hand side of stage r, gathered into the solve vector (new order)
+    for (int q = tid; q < P.c + P.mpad; q += NT) {
+      double v = 0.0;
+      if (q < n) {
+        const int i = P.perm[q];
+        v = forcing_src[i];
+        for (int s = 0; s < r; ++s) {
+          const double coef = p.c[r * 4 + s] / g.H;
+          if (coef != 0.0) {
+            const double ks = dae ? V.K[s * n + i] * A.b.mass[i] : V.K[s * n + i];
+            v = v + coef * ks;
+          }
+        }
+      }
+      vec[q] = v;
+    }
+    solve_cell<TILE>(P, L, a, dLt, dUt);
+    for (int q = tid; q < n; q += NT) V.K[r * n + P.perm[q]] = vec[q];
+    __syncthreads();
+  }
+  // Ynew and the embedded error; E kept in fr.
+  for (int i = tid; i < n; i += NT) {
+    double y = V.y[i], e = 0.0;
+    for (int r = 0; r < S; ++r) {
+      const double kr = V.K[r * n + i];
+      if (p.m[r] != 0.0) y = y + p.m[r] * kr;
+      if (p.e[r] != 0.0) e = e + p.e[r] * kr;
+    }
+    V.w[i] = y;
+    V.fr[i] = e;
+    ynew_out[i] = y;
+  }
+  __syncthreads();
+  return cell_err_sum(V.y, V.w, V.fr, A.b.atol, A.b.rtol, n, V.red);
+}
+
+template <int TILE>
+__global__ __launch_bounds__(NT) void k_batched(BatchArgs A) {
+  extern __shared__ double smem[];
+  const EngineLds L = carve(smem, A.P);
+  const int n = A.M.n, R = A.M.R, S = A.p.stages;
+  CellLds V;
+  V.rd = L.scr;                              // rate work aliases the inverse scratch
+  V.y = smem + A.vec_off;
+  V.f0 = V.y + n;
+  V.fr = V.f0 + n;
+  V.w = V.fr + n;
+  V.K = V.w + n;
+  V.kc = V.K + S * n;
+  V.red = V.kc + R;
+  V.flags = reinterpret_cast<int*>(V.red + 4);
+  __shared__ fode_group_ctrl gs;
+  const int tid = threadIdx.x;
+  const fode_batch_desc& B = A.b;
+  for (;;) {
+    if (tid == 0) V.flags[0] = atomicAdd(B.queue, 1);
+    __syncthreads();
+    const int group = V.flags[0];
+    __syncthreads();
+    if (group >= B.ngroups) break;
+    const int c0 = group * B.group_size;
+    const int cells = min(B.group_size, B.C - c0);
+    if (tid == 0) {
+      gs = B.g[group];
+      gs.dt = B.dt;
+      ctrl_init(A.p, gs);
+      ctrl_outer(A.p, gs);
+    }
+    __syncthreads();
+    while (gs.action != FODE_FINISH) {
+      double sum = 0.0;
+      for (int c = 0; c < cells; ++c) {
+        double* ynew = B.work + (size_t)(c0 + c) * n;
+        const double cs = cell_attempt<TILE>(A, L, V, gs, c0 + c, ynew);
+        sum = sum + cs;
+      }
+      if (tid == 0) ctrl_after(A.p, gs, err_from_sum(sum, (double)cells * n));
+      __syncthreads();
+      if (gs.accepted) {
+        const long N = (long)cells * n;
+        double* Y = B.Y + (size_t)c0 * n;
+        const double* W = B.work + (size_t)c0 * n;
+        for (long i = tid; i < N; i += NT) Y[i] = W[i];
+      }
+      __syncthreads();
+    }
+    // clamp (fastode/solvers.py:89-97) and record the call's status
+    {
+      const long N = (long)cells * n;
+      double* Y = B.Y + (size_t)c0 * n;
+      for (long i = tid; i < N; i += NT) {
+        if (A.p.constrained && B.mass[i % n] != 1.0) continue;
+        Y[i] = fmax(Y[i], 0.0);
+      }
+    }
+    if (tid == 0) {
+      if (gs.status != FODE_CONVERGED) {
+        if (gs.fails == 0) gs.first_fail = (int)gs.counts[FODE_C_CALLS] - 1;
+        gs.fails += 1;
+      }
+      B.g[group] = gs;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_reset_queue(int* q) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *q = 0;
+}
+
+int lds_doubles(const PlanDev& P, const MechDev& M, int S, int* vec_off) {
+  const int eng = engine_lds_doubles(P.lds_len, P.mpad, P.tile);
+  // the rate work (R*P partials, R rates) lives in the engine's scratch
+  const int scr = P.tile * NT;
+  const int need_scr = (M.R * M.P > M.R ? M.R * M.P : M.R);
+  const int extra = need_scr > scr ? need_scr - scr : 0;
+  *vec_off = eng + extra;
+  return eng + extra + cell_region_doubles(M.n, M.R, S);
+}
+
+template <int TILE>
+int launch_batched(BatchArgs A, int blocks_per_cu, hipStream_t s) {
+  int vec_off = 0;
+  const int nd = lds_doubles(A.P, A.M, A.p.stages, &vec_off);
+  A.vec_off = vec_off;
+  const size_t bytes = (size_t)nd * sizeof(double);
+  hipError_t e = hipFuncSetAttribute((const void*)k_batched<TILE>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (e != hipSuccess) return bfail(std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
+  int dev = 0, cus = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  int occ = 0;
+  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_batched<TILE>, NT, bytes);
+  if (e != hipSuccess || occ < 1) occ = 1;
+  const int per_cu = blocks_per_cu > 0 ? blocks_per_cu : occ;
+  int blocks = cus * per_cu;
+  if (blocks > A.b.ngroups) blocks = A.b.ngroups;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, s, A.b.queue);
+  hipLaunchKernelGGL(k_batched<TILE>, dim3(blocks), dim3(NT), bytes, s, A);
+  e = hipGetLastError();
+  if (e != hipSuccess) return bfail(std::string("k_batched: ") + hipGetErrorString(e));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fode_batched_lds_bytes(void* plan, void* mech, int stages, int64_t* bytes) {
+  if (!plan || !mech || !bytes || stages < 1 || stages > 4) return bfail("fode_batched_lds_bytes: bad argument");
+  const PlanDev& P = *static_cast<const PlanDev*>(fode_internal_plan(plan));
+  const MechDev& M = *static_cast<const MechDev*>(fode_internal_mech(mech));
+  int off = 0;
+  *bytes = (int64_t)lds_doubles(P, M, stages, &off) * (int64_t)sizeof(double);
+  return 0;
+}
+
+struct GraphHandle {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+int fode_batched_graph_create(void* plan, void* mech, const fode_ctrl_params* p,
+                              const fode_batch_desc* b, int blocks_per_cu, void** out) {
+  if (!out) return bfail("fode_batched_graph_create: null argument");
+  hipStream_t s = nullptr;
+  hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+  if (e != hipSuccess) return bfail(std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  auto* h = new GraphHandle();
+  int rc = 0;
+  e = hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed);
+  if (e != hipSuccess) {
+    rc = bfail(std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
+  } else {
+    rc = fode_batched_interval(plan, mech, p, b, blocks_per_cu, s);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(s, &g);
+    if (rc == 0 && e != hipSuccess) rc = bfail(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    h->graph = g;
+    if (rc == 0) {
+      e = hipGraphInstantiate(&h->exec, g, nullptr, nullptr, 0);
+      if (e != hipSuccess) rc = bfail(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+  }
+  (void)hipStreamDestroy(s);
+  if (rc != 0) {
+    fode_graph_destroy(h);
+    return rc;
+  }
+  *out = h;
+  return 0;
+}
+
+int fode_graph_launch(void* graph, void* stream) {
+  auto* h = static_cast<GraphHandle*>(graph);
+  if (!h || !h->exec) return bfail("fode_graph_launch: null graph");
+  hipError_t e = hipGraphLaunch(h->exec, reinterpret_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return bfail(std::string("hipGraphLaunch: ") + hipGetErrorString(e));
+  return 0;
+}
+
+int fode_graph_destroy(void* graph) {
+  auto* h = static_cast<GraphHandle*>(graph);
+  if (!h) return 0;
+  if (h->exec) (void)hipGraphExecDestroy(h->exec);
+  if (h->graph) (void)hipGraphDestroy(h->graph);
+  delete h;
+  return 0;
+}
+
+int fode_batched_interval(void* plan, void* mech, const fode_ctrl_params* p, const fode_batch_desc* b,
+                          int blocks_per_cu, void* stream) {
+  if (!plan || !mech || !p || !b) return bfail("fode_batched_interval: null argument");
+  if (b->C <= 0 || b->group_size <= 0 || b->ngroups != (b->C + b->group_size - 1) / b->group_size)
+    return bfail("fode_batched_interval: C, group_size and ngroups disagree");
+  if (p->stages < 1 || p->stages > 4) return bfail("fode_batched_interval: bad tableau");
+  if (!b->Y || !b->k || !b->kmul || !b->atol || !b->mass || !b->g || !b->work || !b->queue)
+    return bfail("fode_batched_interval: null array");
+  BatchArgs A;
+  A.P = *static_cast<const PlanDev*>(fode_internal_plan(plan));
+  A.M = *static_cast<const MechDev*>(fode_internal_mech(mech));
+  if (A.P.n != A.M.n) return bfail("fode_batched_interval: plan and mechanism sizes differ");
+  A.p = *p;
+  A.b = *b;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  switch (A.P.tile) {
+#define X(T) case T: return launch_batched<T>(A, blocks_per_cu, s);
+    X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10)
+#undef X
+  }
+  return bfail("fode_batched_interval: unsupported tile");
+}
+
+}  // extern "C"

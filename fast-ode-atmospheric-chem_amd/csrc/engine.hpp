@@ -527,4 +527,38 @@ __device__ inline double block_sum(double x, double* red) {
   return s;   // valid in thread 0
 }
 
+// One cell's share of error_norm's sum (fastode/solvers.py:37-43):
+//   sum_i (E_i / (atol_i + rtol * max(|Y_i|, |Ynew_i|)))^2,
+// thread t taking i = t, t + NT, ... and then block_sum's tree.  The plain
+// and the batched forms both sum a group cell by cell in this order, so they
+// see the same error.
+__device__ inline double cell_err_sum(const double* Y, const double* Ynew, const double* E,
+                                      const double* atol, double rtol, int n, double* red) {
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += NT) {
+    const double ymax = fmax(fabs(Y[i]), fabs(Ynew[i]));
+    const double s = E[i] / (atol[i] + rtol * ymax);
+    acc = acc + s * s;
+  }
+  return block_sum<NT>(acc, red);
+}
+
+// error_norm's value from the group's sum over size entries.
+__device__ inline double err_from_sum(double sum, double size) {
+  const double v = sqrt(sum / size);
+  return isnan(v) ? v : fmax(v, 1.0e-10);
+}
+
+// Jacobian value of entry e from the rate partials d (R*P) in LDS, the same
+// arithmetic as the standalone Jacobian kernel.
+struct JFromD {
+  MechDev M;
+  const double* d;
+  __device__ double operator()(int e) const {
+    double v = 0.0;
+    for (int q = M.jac_ptr[e]; q < M.jac_ptr[e + 1]; ++q) v = v + M.jac_coef[q] * d[M.jac_src[q]];
+    return v + M.jac_const[e];
+  }
+};
+
 }  // namespace fode

@@ -233,16 +233,14 @@ __global__ __launch_bounds__(256) void k_norm_control(fode_ctrl_params p, fode_g
   __shared__ double red[4];
   __shared__ int flags[2];
   const long N = (long)C * n;
-  double acc = 0.0;
-  for (long i = threadIdx.x; i < N; i += blockDim.x) {
-    const double ymax = fmax(fabs(Y[i]), fabs(Ynew[i]));
-    const double s = E[i] / (atol[i % n] + rtol * ymax);
-    acc = acc + s * s;
+  double sum = 0.0;
+  for (int c = 0; c < C; ++c) {
+    const size_t o = (size_t)c * n;
+    const double cs = cell_err_sum(Y + o, Ynew + o, E + o, atol, rtol, n, red);
+    sum = sum + cs;
   }
-  const double sum = block_sum<256>(acc, red);
   if (threadIdx.x == 0) {
-    const double v = sqrt(sum / (double)N);
-    const double err = isnan(v) ? v : fmax(v, 1.0e-10);
+    const double err = err_from_sum(sum, (double)N);
     fode_group_ctrl c = *g;
     ctrl_after(p, c, err);
     *g = c;
@@ -310,6 +308,11 @@ KPtrs kptrs(const double* const* K, int count) {
 extern "C" {
 
 const char* fode_last_error(void) { return g_err.c_str(); }
+
+// Internal accessors for the other translation units of the library.
+const void* fode_internal_plan(void* plan) { return &static_cast<PlanHandle*>(plan)->d; }
+const void* fode_internal_mech(void* mech) { return &static_cast<MechHandle*>(mech)->d; }
+void fode_internal_set_error(const char* msg) { g_err = msg; }
 
 int fode_device_info(int device, int* lds_per_block, int* cus, int* warp_size) {
   HIP_OK(hipDeviceGetAttribute(lds_per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, device));

@@ -51,6 +51,7 @@ typedef struct fode_group_ctrl {
   double t, H, H_ctrl, proposed, dt, h_min, h_max, err, alpha;
   double h_next, err_prev, h_prev; /* Memory */
   int32_t nsteps, status, last, shortened, rej_last, rej_more, action, accepted;
+  int32_t fails, first_fail;       /* batched form: failed calls, first failing call */
   int64_t counts[FODE_NCOUNTS];
 } fode_group_ctrl;
 
@@ -152,6 +153,45 @@ int fode_norm_control(const fode_ctrl_params* p, fode_group_ctrl* g, int C, int 
 
 /* clamp(system, Y) alone (fastode/solvers.py:89-97). */
 int fode_clamp(int C, int n, double* Y, const double* mass, int constrained, void* stream);
+
+/* ---- Batched persistent form: every group of a batch over one host
+ * interval in one launch.  It replaces the reference's group loop around
+ * Algorithm.solve (fastode/ts1.py:196-216) with identical per-group
+ * semantics: each group (group_size consecutive cells; the last may be
+ * shorter) keeps its own controller state and Memory in g[group] across
+ * calls.  Rate constants of cell c are k[c*k_stride + j] * kmul[j]
+ * (k_stride = 0 shares one row); work is a C*n scratch; queue is one int32
+ * that the call resets itself.  Groups are taken from an atomic queue by
+ * persistent workgroups, so uneven groups balance.  The launch sequence is
+ * stream-ordered and free of host synchronization and memsets, so it may be
+ * captured in a hipGraph. */
+typedef struct fode_batch_desc {
+  int32_t C, group_size, ngroups, k_stride;
+  double dt, rtol;
+  double* Y;               /* (C,n) state, updated in place */
+  const double* k;         /* (C or 1, R) */
+  const double* kmul;      /* (R) per-call multiplier (photolysis factor) */
+  const double* atol;      /* (n) */
+  const double* mass;      /* (n) */
+  fode_group_ctrl* g;      /* (ngroups) */
+  double* work;            /* (C,n) */
+  int32_t* queue;          /* 1 int32 */
+} fode_batch_desc;
+
+/* Launch one interval. blocks_per_cu <= 0 picks the occupancy limit. */
+int fode_batched_interval(void* plan, void* mech, const fode_ctrl_params* p, const fode_batch_desc* b,
+                          int blocks_per_cu, void* stream);
+
+/* LDS bytes one workgroup of the batched kernel uses for this plan/mechanism. */
+int fode_batched_lds_bytes(void* plan, void* mech, int stages, int64_t* bytes);
+
+/* The same interval launch captured once in a hipGraph (queue reset +
+ * persistent kernel) for replay; the arrays named in b are baked in, their
+ * contents (Y, kmul, g, ...) may change between replays. */
+int fode_batched_graph_create(void* plan, void* mech, const fode_ctrl_params* p,
+                              const fode_batch_desc* b, int blocks_per_cu, void** graph);
+int fode_graph_launch(void* graph, void* stream);
+int fode_graph_destroy(void* graph);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,118 @@
+"""GPU: the batched persistent form against the golden vectors and against
+the plain drop-in (identical Counts and bitwise-identical states), eager and
+replayed from its hipGraph."""
+import numpy as np
+import pytest
+import torch
+
+from golden_util import COUNT_NAMES, GOLDEN, load, workload
+
+pytestmark = pytest.mark.gpu
+
+
+def kmul_for(wl, it):
+    from fastode_amd.mechanism import diurnal_factor
+    m = np.ones(wl.R)
+    if wl.diurnal:
+        m[wl.photolysis] = diurnal_factor(it * wl.dt)
+    return m
+
+
+def static_k(wl, cells):
+    k = np.broadcast_to(wl.k_base, (len(cells), wl.R)).copy()
+    if wl.cell_scale is not None:
+        k *= wl.cell_scale[cells]
+    return k
+
+
+def replay_batched(name, graph):
+    import fastode_amd as fa
+    from fastode_amd.system import ChemSystem
+    g = load(name)
+    wl = workload(name)
+    tab = fa.ROS3 if wl.tableau == "ros3" else fa.RODAS3
+    cells = g["cells"]
+    system = ChemSystem(wl.mech, static_k(wl, cells), mass=wl.mass)
+    balg = fa.BatchedAlgorithm(tab)
+    run = balg.runner(system, g["Y0"], wl.dt, wl.atol, wl.rtol, graph=graph)
+    nint = int(g["n_intervals"])
+    prev = run.memory.counts()
+    worst = 0.0
+    for it in range(nint):
+        run.step(kmul_for(wl, it))
+        rec = run.memory.read()
+        cnt = rec["counts"] - prev
+        prev = rec["counts"].copy()
+        for gi in range(len(g["groups"])):
+            assert list(cnt[gi]) == list(g["counts"][it, gi]), (it, gi, list(cnt[gi]), list(g["counts"][it, gi]))
+            assert fa_status(rec["status"][gi]) == g["status"][it, gi]
+        if (it + 1) in list(g["snap_interval"]):
+            s = list(g["snap_interval"]).index(it + 1)
+            ref = g["snaps"][s]
+            Y = run.Y.cpu().numpy()
+            big = np.abs(ref) > 1e-17
+            worst = max(worst, float((np.abs(Y - ref)[big] / np.abs(ref)[big]).max()))
+    mem = rec
+    assert np.allclose(mem["h_next"], g["memory"][:, 0], rtol=1e-12)
+    return worst, run.Y.cpu().numpy()
+
+
+def fa_status(code):
+    import fastode_amd._lib as L
+    return L.STATUS[int(code)]
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batched_matches_golden_mech20(graph):
+    worst, _ = replay_batched("mech20_dense", graph)
+    assert worst < 1e-9
+
+
+def test_batched_matches_golden_mech200_diurnal():
+    worst, _ = replay_batched("mech200_diurnal", True)
+    print("batched mech200 worst rel", worst)
+    assert worst < 1e-9
+
+
+def test_batched_matches_golden_mech200_uneven():
+    if not (GOLDEN / "mech200_uneven.npz").exists():
+        pytest.skip("golden vectors not generated")
+    worst, _ = replay_batched("mech200_uneven", True)
+    assert worst < 1e-9
+
+
+def test_batched_equals_plain_bitwise_and_graph_replays_match_eager():
+    """Groups of uneven size (the last one short), 3 intervals: the batched
+    states equal the plain path's bit for bit, with eager work between graph
+    replays."""
+    import fastode_amd as fa
+    from fastode_amd import workloads as W
+    from fastode_amd.system import ChemSystem
+    wl = W.small(C=80)          # 3 groups: 32, 32, 16
+    k = static_k(wl, np.arange(80))
+    system = ChemSystem(wl.mech, k)
+    balg = fa.BatchedAlgorithm(fa.ROS3)
+    eager = balg.runner(system, wl.Y0, wl.dt, wl.atol, wl.rtol, graph=False)
+    graphed = balg.runner(system, wl.Y0, wl.dt, wl.atol, wl.rtol, graph=True)
+    alg = fa.Algorithm(fa.ROS3)
+    mems = [alg.new_memory() for _ in range(3)]
+    Yp = wl.Y0.copy()
+    for it in range(3):
+        km = kmul_for(wl, it)
+        eager.step(km)
+        junk = torch.randn(1000, device="cuda").sum()      # eager work between replays
+        graphed.step(km)
+        torch.cuda.synchronize()
+        _ = float(junk)
+        for gi, sl in enumerate([slice(0, 32), slice(32, 64), slice(64, 80)]):
+            c = fa.Counts()
+            Yp[sl], st = alg.solve(ChemSystem(wl.mech, wl.k(it)[sl]), Yp[sl], wl.dt, wl.atol, wl.rtol, c,
+                                   mems[gi])
+        Ye = eager.Y.cpu().numpy()
+        Yg = graphed.Y.cpu().numpy()
+        assert np.array_equal(Ye, Yg)
+        assert np.array_equal(Ye, Yp)
+    re, rg = eager.memory.read(), graphed.memory.read()
+    assert np.array_equal(re["counts"], rg["counts"])
+    for gi in range(3):
+        assert re["h_next"][gi] == mems[gi].h_next

@@ -128,6 +128,6 @@ def test_abi_library_exports_every_declared_symbol():
 
 def test_ctrl_record_layout_matches_header():
     import fastode_amd._lib as L
-    # 12 doubles, 8 int32, 13 int64 (include/fastode_hip.h: fode_group_ctrl)
-    assert L.CTRL_DTYPE.itemsize == 12 * 8 + 8 * 4 + 13 * 8
+    # 12 doubles, 10 int32, 13 int64 (include/fastode_hip.h: fode_group_ctrl)
+    assert L.CTRL_DTYPE.itemsize == 12 * 8 + 10 * 4 + 13 * 8
     assert ct.sizeof(L.CtrlParams) == 6 * 8 + 8 * 4 + 16 * 8 * 2 + 4 * 8 * 2 + 4 * 4
