@@ -1,0 +1,232 @@
+"""Throughput of the batched Rosenbrock integrator on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+A step is one host interval of the batched persistent form (one hipGraph
+replay: queue reset + persistent kernel) over every group of the workload:
+the seeded 200-species, 500-reaction mechanism with diurnal photolysis on
+65,536 cells in groups of 32, ROS3, rtol 1e-3, atol 1e-20, 600 s intervals
+(48 of them by default).  Warmup intervals run on a separate copy of the
+batch; the timed run starts from the initial state.  Inputs are resident in
+HBM before timing starts.
+
+The JSON line also carries the per-group plain form (one launch sequence
+per group and protocol call, the reference's call structure) measured on a
+sample of groups over the same intervals, the HBM roofline of the interval
+kernel and the CPU oracle's rate on the box's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "cell-intervals/s, batched vs per-group launch, 200 spp rtol 1e-3; adjoint/fwd"
+HBM_PEAK_GBS = 8000.0
+V_PASSES = 22      # unfused stage/norm vector passes per attempt counted per species
+
+
+def algorithmic_bytes(counts, cells_per_group, n, R, nnz_j, nnz_lu):
+    """Compulsory bytes of a streaming implementation, summed over groups:
+    8 * cells * [lu (nnz_J + nnz_LU) + solves (nnz_LU + 2n) + jac (n + R + nnz_J)
+                 + rhs (2n + R) + steps V n]."""
+    c = counts.astype(np.float64)
+    steps, rhs, jac, lu, solves = c[:, 0], c[:, 3], c[:, 4], c[:, 5], c[:, 6]
+    per_cell = (lu * (nnz_j + nnz_lu) + solves * (nnz_lu + 2 * n) + jac * (n + R + nnz_j)
+                + rhs * (2 * n + R) + steps * V_PASSES * n)
+    return float(8.0 * np.sum(per_cell * cells_per_group))
+
+
+def algorithmic_flops(counts, cells_per_group, n, R, P, nnz_j, fact_flops, nnz_lu, jac_terms, rhs_terms):
+    c = counts.astype(np.float64)
+    steps, rhs, jac, lu, solves = c[:, 0], c[:, 3], c[:, 4], c[:, 5], c[:, 6]
+    f_rhs = R * P + 2.0 * rhs_terms
+    f_jac = R * P * (P - 1) + 2.0 * jac_terms
+    f_vec = 27.0 * n
+    per_cell = lu * fact_flops + solves * 2.0 * nnz_lu + jac * f_jac + rhs * f_rhs + steps * f_vec
+    return float(np.sum(per_cell * cells_per_group))
+
+
+def _oracle_job(args):
+    """One group of the CPU sample through the oracle (runs in a worker)."""
+    os.environ.setdefault("OMP_NUM_THREADS", "1")
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import fastode_oracle as O
+    import __graft_entry__
+    __graft_entry__.load_package()
+    from fastode_amd import workloads as W
+    C, group, n_int = args
+    wl = W.diurnal200(C=C)
+    mech = wl.mech
+    om = O.OracleMechanism(mech.species, mech.reactions_list())
+    cells = np.arange(group * 32, group * 32 + 32)
+    alg = O.Algorithm(O.ROS3)
+    mem = alg.new_memory()
+    Y = wl.Y0[cells].copy()
+    for it in range(n_int):
+        Y, _ = alg.solve(O.ChemSystem(om, wl.k(it, cells), dense=False), Y, wl.dt, wl.atol, wl.rtol,
+                         O.Counts(), mem)
+    return 32 * n_int
+
+
+def cpu_baseline(C, n_groups, n_int):
+    import multiprocessing as mp
+    cores = max(1, min(16, len(os.sched_getaffinity(0))))
+    os.environ["OMP_NUM_THREADS"] = "1"
+    os.environ["OPENBLAS_NUM_THREADS"] = "1"
+    ctx = mp.get_context("spawn")
+    t0 = time.perf_counter()
+    with ctx.Pool(cores) as pool:
+        done = sum(pool.map(_oracle_job, [(min(C, 32 * n_groups), g, n_int) for g in range(n_groups)]))
+    wall = time.perf_counter() - t0
+    return {"value": done / wall, "unit": "cell-intervals/s", "cores": cores, "kind": "port",
+            "sample": f"oracle (NumPy/SciPy SuperLU restatement), first {n_groups} groups x {n_int} "
+                      f"intervals = {done} cell-intervals in {wall:.1f} s, spawn pool, 1 BLAS thread each"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=48)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cells", type=int, default=65536)
+    ap.add_argument("--plain-groups", type=int, default=8)
+    ap.add_argument("--cpu-groups", type=int, default=0, help="0: 2 per core")
+    ap.add_argument("--cpu-intervals", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--blocks-per-cu", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: the groups shard "
+              "across GPUs with no exchange, so the figure is N times the one-GPU rate", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import __graft_entry__
+    fa = __graft_entry__.load_package()
+    from fastode_amd import workloads as W
+    from fastode_amd.mechanism import diurnal_factor
+    from fastode_amd.system import ChemSystem
+    from fastode_amd.device import mechanism_plan
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    wl = W.diurnal200(C=args.cells, n_intervals=args.steps)
+    mech = wl.mech
+    K = args.steps
+
+    def kmul(it):
+        m = np.ones(wl.R)
+        m[wl.photolysis] = diurnal_factor(it * wl.dt)
+        return m
+
+    system = ChemSystem(mech, wl.k_base[None, :])
+    balg = fa.BatchedAlgorithm(fa.ROS3, blocks_per_cu=args.blocks_per_cu)
+    plan = mechanism_plan(mech).plan
+
+    # warmup on a copy of the batch
+    warm = balg.runner(system, wl.Y0, wl.dt, wl.atol, wl.rtol, graph=True)
+    for it in range(args.warmup):
+        warm.step(kmul(it))
+    torch.cuda.synchronize()
+    del warm
+
+    run = balg.runner(system, wl.Y0, wl.dt, wl.atol, wl.rtol, graph=True)
+    stream = torch.cuda.current_stream()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(K)]
+    kms = [torch.as_tensor(kmul(it), dtype=torch.float64, device=dev) for it in range(K)]
+    counts_before = run.memory.counts()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for it in range(K):
+        run.kmul.copy_(kms[it])
+        ev[it][0].record(stream)
+        run.step_graph_only()
+        ev[it][1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    launch_ms = [a.elapsed_time(b) for a, b in ev]
+    kernel_s = sum(launch_ms) / 1e3
+    rec = run.memory.read()
+    counts = rec["counts"] - counts_before
+    cells_per_group = np.minimum(32, wl.C - 32 * np.arange(run.ngroups))
+    fails = int((rec["fails"] > 0).sum())
+
+    cell_intervals = wl.C * K
+    value = cell_intervals / wall
+    fact_flops, nnz_lu = plan.flops()
+    fact_flops = 2 * fact_flops
+    alg_bytes = algorithmic_bytes(counts, cells_per_group, wl.n, wl.R, mech.nnz, nnz_lu)
+    alg_flops = algorithmic_flops(counts, cells_per_group, wl.n, wl.R, mech.P, mech.nnz, fact_flops, nnz_lu,
+                                  len(mech.jac_src), len(mech.rhs_rxn))
+    achieved = alg_bytes / kernel_s / 1e9
+    traffic = None
+    pmc = ROOT / "profiles" / "traffic_mech200-diurnal.json"
+    if pmc.exists():
+        t = json.loads(pmc.read_text())
+        traffic = t.get("hbm_bytes_per_launch")
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "fp64_tflops": round(alg_flops / kernel_s / 1e12, 3), "fp64_frac": round(
+                    alg_flops / kernel_s / 78.6e12, 4),
+                "alg_bytes_per_launch": round(alg_bytes / K), "avg_launch_ms": round(1e3 * kernel_s / K, 3)}
+
+    out = {"metric": METRIC, "value": round(value, 1), "unit": "cell-intervals/s", "n_gpus": 1,
+           "steps": K, "warmup": args.warmup, "ms_per_step": round(1e3 * wall / K, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (seeded mechanism and states)",
+           "config": {"workload": "mech200-diurnal", "species": wl.n, "reactions": wl.R, "cells": wl.C,
+                      "group_size": 32, "groups": run.ngroups, "intervals": K, "dt_s": wl.dt,
+                      "rtol": wl.rtol, "atol": float(wl.atol[0]), "tableau": "ros3",
+                      "form": "batched persistent kernel, one hipGraph replay per interval",
+                      "lu": {"nnz_J": mech.nnz, "nnz_LU": nnz_lu, "core": plan.m, "tile": plan.tile}},
+           "attempts_per_group_interval": round(float(counts[:, 0].sum()) / (run.ngroups * K), 3),
+           "rejected_per_group_interval": round(float(counts[:, 2].sum()) / (run.ngroups * K), 3),
+           "failed_groups": fails,
+           "roofline": roofline}
+
+    if not args.no_plain:
+        alg = fa.Algorithm(fa.ROS3)
+        G = min(args.plain_groups, run.ngroups)
+        Yp = wl.Y0[: 32 * G].copy()
+        mems = [alg.new_memory() for _ in range(G)]
+        pc = fa.Counts()
+        systems = [[None] * G for _ in range(K)]
+        for it in range(K):
+            for g in range(G):
+                systems[it][g] = ChemSystem(mech, wl.k(it, np.arange(32 * g, 32 * g + 32)))
+        Yd = torch.as_tensor(Yp, device=dev)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for it in range(K):
+            for g in range(G):
+                sl = slice(32 * g, 32 * g + 32)
+                Yd[sl], _ = alg.solve(systems[it][g], Yd[sl], wl.dt, wl.atol, wl.rtol, pc, mems[g])
+        torch.cuda.synchronize()
+        pwall = time.perf_counter() - t1
+        plain_value = 32 * G * K / pwall
+        same = bool(np.array_equal(Yd.cpu().numpy(), run.Y[: 32 * G].cpu().numpy()))
+        out["plain"] = {"value": round(plain_value, 1), "unit": "cell-intervals/s",
+                        "sample": f"first {G} groups x {K} intervals, one launch sequence per group and call",
+                        "attempts_per_group_interval": round(pc.steps / (G * K), 3),
+                        "states_bitwise_equal_to_batched": same}
+        out["speedup_vs_plain"] = round(value / plain_value, 1)
+
+    if not args.no_cpu:
+        cores = max(1, min(16, len(os.sched_getaffinity(0))))
+        ng = args.cpu_groups or 2 * cores
+        out["cpu_baseline"] = cpu_baseline(wl.C, ng, args.cpu_intervals)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

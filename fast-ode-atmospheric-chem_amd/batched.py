@@ -127,6 +127,10 @@ class BatchedRunner:
 
     def step(self, kmul=None):
         self.set_kmul(kmul)
+        self.step_graph_only()
+
+    def step_graph_only(self):
+        """Advance one interval with the multipliers already in ``self.kmul``."""
         s = L.stream_handle()
         if self.graph is not None:
             L.check(L.lib.fode_graph_launch(self.graph, s), "fode_graph_launch")

@@ -102,10 +102,16 @@ class ChemSystem(_Base):
         self._plan = mechanism_plan(mech)
 
     def _k_for(self, C):
+        """k with one row per cell; a shared row is expanded once and kept, so
+        the kernels never read a freed temporary."""
         if self.k.shape[0] == C:
             return self.k
         if self.k.shape[0] == 1:
-            return self.k.expand(C, -1).contiguous()
+            kc = getattr(self, "_k_expanded", None)
+            if kc is None or kc.shape[0] != C:
+                kc = self.k.expand(C, -1).contiguous()
+                self._k_expanded = kc
+            return kc
         raise ValueError(f"k has {self.k.shape[0]} rows for {C} cells")
 
     def rhs(self, Y, out=None):

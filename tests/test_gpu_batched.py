@@ -51,7 +51,11 @@ def replay_batched(name, graph):
             ref = g["snaps"][s]
             Y = run.Y.cpu().numpy()
             big = np.abs(ref) > 1e-17
-            worst = max(worst, float((np.abs(Y - ref)[big] / np.abs(ref)[big]).max()))
+            rel = np.where(big, np.abs(Y - ref) / np.maximum(np.abs(ref), 1e-300), 0.0)
+            at = np.unravel_index(np.argmax(rel), rel.shape)
+            print(f"{name} interval {it + 1}: worst rel {rel.max():.3e} at value {ref[at]:.3e} cell {at[0]} "
+                  f"species {at[1]}")
+            worst = max(worst, float(rel.max()))
     mem = rec
     assert np.allclose(mem["h_next"], g["memory"][:, 0], rtol=1e-12)
     return worst, run.Y.cpu().numpy()
