@@ -9,6 +9,10 @@ import ctypes as ct
 from pathlib import Path
 
 import numpy as np
+# torch first: its HIP runtime (soname libamdhip64.so.7) must be the one the
+# library binds to, so that torch's streams and allocations and the library's
+# kernels live in one runtime instance.
+import torch  # noqa: F401
 
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "libfastode_hip.so"
@@ -61,13 +65,13 @@ class MechDesc(ct.Structure):
 
 class PlanDesc(ct.Structure):
     _fields_ = [("n", ct.c_int32), ("c", ct.c_int32), ("m", ct.c_int32), ("tile", ct.c_int32),
-                ("ns", ct.c_int32), ("nnz", ct.c_int32),
-                ("f_ntasks", ct.c_int32), ("f_nst", ct.c_int32), ("f_npairs", ct.c_int32),
-                ("schur_rows", ct.c_int32),
-                ("sf_ntasks", ct.c_int32), ("sf_nst", ct.c_int32), ("sb_ntasks", ct.c_int32),
-                ("sb_nst", ct.c_int32), ("s_npairs", ct.c_int32),
+                ("ns", ct.c_int32), ("nnz", ct.c_int32), ("slot_stride", ct.c_int32),
+                ("n_f_tasks", ct.c_int32), ("f_nst", ct.c_int32), ("n_f_pairs", ct.c_int32),
+                ("n_schur", ct.c_int32),
+                ("n_sf_tasks", ct.c_int32), ("sf_nst", ct.c_int32), ("n_sb_tasks", ct.c_int32),
+                ("sb_nst", ct.c_int32), ("n_s_pairs", ct.c_int32),
                 ("sp_src", ct.c_void_p), ("sp_diag", ct.c_void_p), ("sp_row", ct.c_void_p),
-                ("core_src", ct.c_void_p), ("core_diag", ct.c_void_p), ("core_row", ct.c_void_p),
+                ("core_src", ct.c_void_p), ("core_row", ct.c_void_p),
                 ("f_tasks", ct.c_void_p), ("f_ptr", ct.c_void_p), ("f_pairs", ct.c_void_p),
                 ("schur_ptr", ct.c_void_p), ("schur", ct.c_void_p),
                 ("sf_tasks", ct.c_void_p), ("sf_ptr", ct.c_void_p),

@@ -2,14 +2,15 @@
 // one host interval.
 //
 // Workgroups (256 threads) are persistent and take groups from an atomic
-// queue.  A workgroup runs its group's whole step loop (the controller of
-// Algorithm.solve, fastode/solvers.py:305-415, on the device) and performs
-// every attempt cell by cell: the cell's rates, Jacobian, stage matrix and
-// its LU, the stage solves and the stage combination all stay in LDS and
-// registers; only the cell's state, rate constants and the new state touch
-// memory.  The group's error is the sum of the cells' partial sums in cell
-// order, exactly as the plain form forms it, so both forms take the same
-// steps.
+// queue.  A workgroup first copies the plan's task tables and Schur lists
+// into its LDS (they are read by every attempt of every cell), then runs
+// each group's whole step loop (the controller of Algorithm.solve,
+// fastode/solvers.py:305-415, on the device) and performs every attempt cell
+// by cell: the cell's rates, Jacobian, stage matrix and its LU, the stage
+// solves and the stage combination all stay in LDS and registers; only the
+// cell's state, rate constants and new state touch memory.  The group's error
+// is the sum of the cells' partial sums in cell order, exactly as the plain
+// form forms it, so both forms take the same steps.
 #include <hip/hip_runtime.h>
 
 #include <string>
@@ -17,10 +18,6 @@
 #include "engine.hpp"
 
 using namespace fode;
-
-namespace {
-
-}  // namespace
 
 // Shared with fastode_hip.hip, which owns the handles and the error text.
 extern "C" const void* fode_internal_plan(void* plan);
@@ -34,32 +31,74 @@ int bfail(const std::string& m) {
   return -1;
 }
 
+// LDS layout of the batched kernel, in doubles from the dynamic base.
+struct Layout {
+  int tables;      // uint32 words of the staged tables (rounded to doubles)
+  int engine;      // engine region offset
+  int cell;        // cell region offset
+  int rd;          // rates / rate partials: the inverse scratch when it fits
+  int jval;        // Jacobian values: the inverse-block region (dead until factored)
+  int total;
+};
+
 struct BatchArgs {
   PlanDev P;
   MechDev M;
   fode_ctrl_params p;
   fode_batch_desc b;
-  int vec_off;    // LDS offsets (doubles) of the cell region
-};
-
-// LDS of the cell region, after the engine region.
-struct CellLds {
-  double* y;      // n
-  double* f0;     // n   F(Y)
-  double* fr;     // n   forcing of a later stage
-  double* w;      // n   stage point / new state
-  double* K;      // S*n
-  double* kc;     // R   rate constants
-  double* rd;     // max(R*P, R): rate partials, rates (aliases the engine's scr)
-  double* red;    // 4
-  int* flags;     // 4 ints
+  Layout lay;
 };
 
 __host__ __device__ inline int cell_region_doubles(int n, int R, int S) {
   return 4 * n + S * n + R + 4 + 2;
 }
 
-template <int TILE>
+Layout make_layout(const PlanDev& P, const MechDev& M, int S) {
+  Layout l;
+  const int words = table_words(P.n_f_tasks, P.f_nst, P.n_f_pairs, P.n_sf_tasks, P.sf_nst,
+                                P.n_sb_tasks, P.sb_nst, P.n_s_pairs, P.n_schur);
+  l.tables = (words + 1) / 2;
+  l.engine = l.tables;
+  int end = l.engine + engine_lds_doubles(P.lds_len, P.mpad, P.tile);
+  // Rates (R) and rate partials (R*P) are needed between the solves, when
+  // only the inverse scratch (scr) is dead; they get a region of their own if
+  // scr is too small.
+  const int scr = P.tile * NT;
+  const int scr_at = l.engine + (P.lds_len + 4 * P.mpad + 2 + 2 * NT);
+  const int need_rd = M.R * M.P > M.R ? M.R * M.P : M.R;
+  if (need_rd <= scr) {
+    l.rd = scr_at;
+  } else {
+    l.rd = end;
+    end += need_rd;
+  }
+  // Jacobian values are dead once assembled, before the inverse blocks
+  // (dL, dU) are written.
+  l.jval = scr_at + scr;
+  if (l.jval + M.nnz > end) end = l.jval + M.nnz;
+  l.cell = end;
+  l.total = l.cell + cell_region_doubles(M.n, M.R, S);
+  return l;
+}
+
+struct CellLds {
+  double* y;      // n
+  double* f0;     // n   F(Y)
+  double* fr;     // n   forcing of a later stage; E at the end
+  double* w;      // n   stage point; the new state at the end
+  double* K;      // S*n
+  double* kc;     // R   rate constants
+  double* rd;     // R*P rate partials / R rates
+  double* jv;     // nnz Jacobian values
+  double* red;    // 4
+  int* flags;     // 4 ints
+};
+
+struct JLds {
+  const double* J;
+  __device__ double operator()(int e) const { return J[e]; }
+};
+
 __device__ inline void rates_into(const MechDev& M, const double* y, const double* kc, double* rates) {
   for (int j = threadIdx.x; j < M.R; j += NT) rates[j] = rate(M, y, kc, j);
   __syncthreads();
@@ -72,8 +111,8 @@ __device__ inline void forcing_into(const MechDev& M, const double* y, const dou
 
 // One attempt on one cell. Returns the cell's error partial sum (thread 0).
 template <int TILE>
-__device__ double cell_attempt(const BatchArgs& A, const EngineLds& L, const CellLds& V,
-                               const fode_group_ctrl& g, int cell, double* ynew_out) {
+__device__ double cell_attempt(const BatchArgs& A, const Tables& T, const EngineLds& L,
+                               const CellLds& V, const fode_group_ctrl& g, int cell, double* ynew_out) {
   const PlanDev& P = A.P;
   const MechDev& M = A.M;
   const fode_ctrl_params& p = A.p;
@@ -83,16 +122,20 @@ __device__ double cell_attempt(const BatchArgs& A, const EngineLds& L, const Cel
   for (int i = tid; i < n; i += NT) V.y[i] = yg[i];
   for (int j = tid; j < M.R; j += NT) V.kc[j] = kg[j] * A.b.kmul[j];
   __syncthreads();
-  // F(Y) (fastode/solvers.py:345) and the Jacobian's rate partials (:347).
-  rates_into<TILE>(M, V.y, V.kc, V.rd);
+  // F(Y) (fastode/solvers.py:345) and the Jacobian (:347), recomputed per
+  // attempt: they are the same values a retry reuses.
+  rates_into(M, V.y, V.kc, V.rd);
   forcing_into(M, V.y, V.rd, V.f0);
   for (int jp = tid; jp < M.R * M.P; jp += NT) V.rd[jp] = drate(M, V.y, V.kc, jp / M.P, jp % M.P);
   __syncthreads();
-  double a[TILE][TILE], dLt[TILE], dUt[TILE];
-  assemble<TILE>(P, JFromD{M, V.rd}, g.alpha, A.b.mass, false, L.lds, a);
-  factor_cell<TILE>(P, L, a, dLt, dUt);
+  const JFromD jd{M, V.rd};
+  for (int e = tid; e < M.nnz; e += NT) V.jv[e] = jd(e);
+  __syncthreads();
+  double a[TILE][TILE];
+  assemble<TILE>(P, JLds{V.jv}, g.alpha, A.b.mass, false, L.lds, a);
+  factor_cell<TILE>(P, T, L, a);
   // Stages (fastode/solvers.py:54-86).
-  double* vec = L.lds + P.ns + 1;
+  double* vec = L.lds + P.ns;
   const int S = p.stages;
   const bool dae = p.constrained != 0;
   const double* forcing_src = V.f0;
@@ -107,7 +150,7 @@ __device__ double cell_attempt(const BatchArgs& A, const EngineLds& L, const Cel
         V.w[i] = v;
       }
       __syncthreads();
-      rates_into<TILE>(M, V.w, V.kc, V.rd);
+      rates_into(M, V.w, V.kc, V.rd);
       forcing_into(M, V.w, V.rd, V.fr);
       forcing_src = V.fr;
     }
@@ -127,7 +170,7 @@ __device__ double cell_attempt(const BatchArgs& A, const EngineLds& L, const Cel
       }
       vec[q] = v;
     }
-    solve_cell<TILE>(P, L, a, dLt, dUt);
+    solve_cell<TILE>(P, T, L, a);
     for (int q = tid; q < n; q += NT) V.K[r * n + P.perm[q]] = vec[q];
     __syncthreads();
   }
@@ -150,17 +193,19 @@ __device__ double cell_attempt(const BatchArgs& A, const EngineLds& L, const Cel
 template <int TILE>
 __global__ __launch_bounds__(NT) void k_batched(BatchArgs A) {
   extern __shared__ double smem[];
-  const EngineLds L = carve(smem, A.P);
-  const int n = A.M.n, R = A.M.R, S = A.p.stages;
+  const int n = A.M.n, S = A.p.stages;
+  const Tables T = stage_tables(A.P, reinterpret_cast<uint32_t*>(smem));
+  const EngineLds L = carve(smem + A.lay.engine, A.P);
   CellLds V;
-  V.rd = L.scr;                              // rate work aliases the inverse scratch
-  V.y = smem + A.vec_off;
+  V.rd = smem + A.lay.rd;
+  V.jv = smem + A.lay.jval;
+  V.y = smem + A.lay.cell;
   V.f0 = V.y + n;
   V.fr = V.f0 + n;
   V.w = V.fr + n;
   V.K = V.w + n;
   V.kc = V.K + S * n;
-  V.red = V.kc + R;
+  V.red = V.kc + A.M.R;
   V.flags = reinterpret_cast<int*>(V.red + 4);
   __shared__ fode_group_ctrl gs;
   const int tid = threadIdx.x;
@@ -184,7 +229,7 @@ __global__ __launch_bounds__(NT) void k_batched(BatchArgs A) {
       double sum = 0.0;
       for (int c = 0; c < cells; ++c) {
         double* ynew = B.work + (size_t)(c0 + c) * n;
-        const double cs = cell_attempt<TILE>(A, L, V, gs, c0 + c, ynew);
+        const double cs = cell_attempt<TILE>(A, T, L, V, gs, c0 + c, ynew);
         sum = sum + cs;
       }
       if (tid == 0) ctrl_after(A.p, gs, err_from_sum(sum, (double)cells * n));
@@ -221,28 +266,19 @@ __global__ void k_reset_queue(int* q) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *q = 0;
 }
 
-int lds_doubles(const PlanDev& P, const MechDev& M, int S, int* vec_off) {
-  const int eng = engine_lds_doubles(P.lds_len, P.mpad, P.tile);
-  // the rate work (R*P partials, R rates) lives in the engine's scratch
-  const int scr = P.tile * NT;
-  const int need_scr = (M.R * M.P > M.R ? M.R * M.P : M.R);
-  const int extra = need_scr > scr ? need_scr - scr : 0;
-  *vec_off = eng + extra;
-  return eng + extra + cell_region_doubles(M.n, M.R, S);
-}
-
 template <int TILE>
 int launch_batched(BatchArgs A, int blocks_per_cu, hipStream_t s) {
-  int vec_off = 0;
-  const int nd = lds_doubles(A.P, A.M, A.p.stages, &vec_off);
-  A.vec_off = vec_off;
-  const size_t bytes = (size_t)nd * sizeof(double);
+  const size_t bytes = (size_t)A.lay.total * sizeof(double);
+  int dev = 0, cus = 0, maxlds = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  (void)hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+  if (bytes + sizeof(fode_group_ctrl) > (size_t)maxlds)
+    return bfail("fode_batched_interval: the plan needs " + std::to_string(bytes) + " bytes of LDS, above " +
+                 std::to_string(maxlds));
   hipError_t e = hipFuncSetAttribute((const void*)k_batched<TILE>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
   if (e != hipSuccess) return bfail(std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
-  int dev = 0, cus = 0;
-  (void)hipGetDevice(&dev);
-  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   int occ = 0;
   e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_batched<TILE>, NT, bytes);
   if (e != hipSuccess || occ < 1) occ = 1;
@@ -265,8 +301,7 @@ int fode_batched_lds_bytes(void* plan, void* mech, int stages, int64_t* bytes) {
   if (!plan || !mech || !bytes || stages < 1 || stages > 4) return bfail("fode_batched_lds_bytes: bad argument");
   const PlanDev& P = *static_cast<const PlanDev*>(fode_internal_plan(plan));
   const MechDev& M = *static_cast<const MechDev*>(fode_internal_mech(mech));
-  int off = 0;
-  *bytes = (int64_t)lds_doubles(P, M, stages, &off) * (int64_t)sizeof(double);
+  *bytes = (int64_t)make_layout(P, M, stages).total * (int64_t)sizeof(double);
   return 0;
 }
 
@@ -334,9 +369,10 @@ int fode_batched_interval(void* plan, void* mech, const fode_ctrl_params* p, con
   BatchArgs A;
   A.P = *static_cast<const PlanDev*>(fode_internal_plan(plan));
   A.M = *static_cast<const MechDev*>(fode_internal_mech(mech));
-  if (A.P.n != A.M.n) return bfail("fode_batched_interval: plan and mechanism sizes differ");
+  if (A.P.n != A.M.n || A.P.nnz != A.M.nnz) return bfail("fode_batched_interval: plan and mechanism differ");
   A.p = *p;
   A.b = *b;
+  A.lay = make_layout(A.P, A.M, p->stages);
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   switch (A.P.tile) {
 #define X(T) case T: return launch_batched<T>(A, blocks_per_cu, s);

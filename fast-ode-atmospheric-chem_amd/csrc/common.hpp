@@ -1,0 +1,49 @@
+// Device-side descriptions of a mechanism and of a symbolic LU plan, shared
+// by every kernel of the library (gfx950, wave64, FP64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/fastode_hip.h"
+
+namespace fode {
+
+constexpr int G = 16;
+constexpr int NT = G * G;   // threads of one cell engine
+constexpr double EPS = 2.220446049250313e-16;   // float64 machine epsilon
+constexpr uint32_t NO_DIV = 0xFFFFu;
+
+// Level-scheduled task tables and the Schur lists of a plan.  The pointers
+// address global memory (plain kernels) or a workgroup's LDS copy (the
+// persistent batched kernel copies them once per workgroup).
+struct Tables {
+  const uint2* f_tasks; const int* f_ptr; const uint32_t* f_pairs;
+  const uint2* sf_tasks; const int* sf_ptr;
+  const uint2* sb_tasks; const int* sb_ptr;
+  const uint32_t* s_pairs;
+  const int* schur_ptr;        // NT + 1
+  const uint32_t* schur;       // slot << 24 | a << 12 | b
+};
+
+struct PlanDev {
+  int n, c, m, tile, ns, nnz;
+  int lds_len, lu_len, mpad, slot_stride;
+  int f_nst, sf_nst, sb_nst;
+  // sizes (uint32 words) of the table arrays, for staging them into LDS
+  int n_f_tasks, n_f_pairs, n_sf_tasks, n_sb_tasks, n_s_pairs, n_schur;
+  Tables t;                    // global-memory tables
+  const int* sp_src; const int* sp_diag; const int* sp_row;
+  const short* core_src;       // [NT][slot_stride]
+  const int* core_row;         // [mpad] old row of core row, -1 padding
+  const int* perm;             // new -> old
+};
+
+struct MechDev {
+  int n, R, P, nnz, ncon;
+  const int* react; const int* rhs_ptr; const int* rhs_rxn; const double* rhs_coef;
+  const int* jac_ptr; const int* jac_src; const double* jac_coef; const double* jac_const;
+  const int* con_rows; const double* con_coef; const double* con_total;
+  const int* con_of_row;   // n: constraint index of a row or -1
+};
+
+}  // namespace fode

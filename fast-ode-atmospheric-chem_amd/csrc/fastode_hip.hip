@@ -107,9 +107,9 @@ __global__ __launch_bounds__(NT) void k_factor(PlanDev P, const double* __restri
   const EngineLds L = carve(smem, P);
   const int cell = blockIdx.x, tid = threadIdx.x;
   const double alpha = *alpha_p;
-  double a[TILE][TILE], dLt[TILE], dUt[TILE];
+  double a[TILE][TILE];
   assemble<TILE>(P, JGlobal{J + (size_t)cell * P.nnz}, alpha, mass, alg_only != 0, L.lds, a);
-  factor_cell<TILE>(P, L, a, dLt, dUt);
+  factor_cell<TILE>(P, P.t, L, a);
   double* out = LU + (size_t)cell * P.lu_len;
   for (int s = tid; s < P.ns; s += NT) out[s] = L.lds[s];
   double* core = out + P.ns;
@@ -118,11 +118,7 @@ __global__ __launch_bounds__(NT) void k_factor(PlanDev P, const double* __restri
 #pragma unroll
     for (int tj = 0; tj < TILE; ++tj) core[(ti * TILE + tj) * NT + tid] = a[ti][tj];
   double* dd = core + TILE * TILE * NT;
-#pragma unroll
-  for (int J2 = 0; J2 < TILE; ++J2) {
-    dd[J2 * NT + tid] = dLt[J2];
-    dd[(TILE + J2) * NT + tid] = dUt[J2];
-  }
+  for (int i = tid; i < 2 * TILE * NT; i += NT) dd[i] = L.dL[i];   // dL and dU are adjacent
 }
 
 template <int TILE>
@@ -132,24 +128,19 @@ __global__ __launch_bounds__(NT) void k_solve(PlanDev P, const double* __restric
   const EngineLds L = carve(smem, P);
   const int cell = blockIdx.x, tid = threadIdx.x;
   const double* in = LU + (size_t)cell * P.lu_len;
-  double a[TILE][TILE], dLt[TILE], dUt[TILE];
+  double a[TILE][TILE];
   for (int s = tid; s < P.ns; s += NT) L.lds[s] = in[s];
-  if (tid == 0) L.lds[P.ns] = 0.0;
   const double* core = in + P.ns;
 #pragma unroll
   for (int ti = 0; ti < TILE; ++ti)
 #pragma unroll
     for (int tj = 0; tj < TILE; ++tj) a[ti][tj] = core[(ti * TILE + tj) * NT + tid];
   const double* dd = core + TILE * TILE * NT;
-#pragma unroll
-  for (int J2 = 0; J2 < TILE; ++J2) {
-    dLt[J2] = dd[J2 * NT + tid];
-    dUt[J2] = dd[(TILE + J2) * NT + tid];
-  }
-  double* vec = L.lds + P.ns + 1;
+  for (int i = tid; i < 2 * TILE * NT; i += NT) L.dL[i] = dd[i];
+  double* vec = L.lds + P.ns;
   const double* b = B + (size_t)cell * P.n;
   for (int i = tid; i < P.c + P.mpad; i += NT) vec[i] = (i < P.n) ? b[P.perm[i]] : 0.0;
-  solve_cell<TILE>(P, L, a, dLt, dUt);
+  solve_cell<TILE>(P, P.t, L, a);
   double* x = X + (size_t)cell * P.n;
   for (int i = tid; i < P.n; i += NT) x[P.perm[i]] = vec[i];
 }
@@ -368,31 +359,33 @@ int fode_plan_create(const fode_plan_desc* d, void** out, int64_t* lu_len) {
   if (!d || !out) return fail("fode_plan_create: null argument");
   if (d->tile < 1 || d->tile > 10) return fail("fode_plan_create: tile must be 1..10");
   if (d->m > G * d->tile || d->c + d->m != d->n) return fail("fode_plan_create: bad core split");
+  if (d->slot_stride < d->tile * d->tile || d->slot_stride % 8) return fail("fode_plan_create: bad slot stride");
   auto* h = new PlanHandle();
   PlanDev& p = h->d;
   p.n = d->n; p.c = d->c; p.m = d->m; p.tile = d->tile; p.ns = d->ns; p.nnz = d->nnz;
   p.mpad = G * d->tile;
-  p.lds_len = d->ns + 1 + d->c + p.mpad;
+  p.slot_stride = d->slot_stride;
+  p.lds_len = d->ns + d->c + p.mpad;
   p.lu_len = d->ns + p.mpad * p.mpad + 2 * d->tile * NT;
   p.f_nst = d->f_nst; p.sf_nst = d->sf_nst; p.sb_nst = d->sb_nst;
-  const size_t nslot = (size_t)d->tile * d->tile * NT;
+  p.n_f_tasks = d->n_f_tasks; p.n_f_pairs = d->n_f_pairs; p.n_sf_tasks = d->n_sf_tasks;
+  p.n_sb_tasks = d->n_sb_tasks; p.n_s_pairs = d->n_s_pairs; p.n_schur = d->n_schur;
   int rc = 0;
   rc |= upload(d->sp_src, (size_t)d->ns, (int**)&p.sp_src, h->owned);
   rc |= upload(d->sp_diag, (size_t)d->ns, (int**)&p.sp_diag, h->owned);
   rc |= upload(d->sp_row, (size_t)d->ns, (int**)&p.sp_row, h->owned);
-  rc |= upload(d->core_src, nslot, (int**)&p.core_src, h->owned);
-  rc |= upload(d->core_diag, nslot, (int**)&p.core_diag, h->owned);
-  rc |= upload(d->core_row, nslot, (int**)&p.core_row, h->owned);
-  rc |= upload((const int4*)d->f_tasks, (size_t)d->f_ntasks, (int4**)&p.f_tasks, h->owned);
-  rc |= upload(d->f_ptr, (size_t)d->f_nst + 1, (int**)&p.f_ptr, h->owned);
-  rc |= upload(d->f_pairs, (size_t)d->f_npairs, (uint32_t**)&p.f_pairs, h->owned);
-  rc |= upload(d->schur_ptr, (size_t)d->tile * d->tile + 1, (int**)&p.schur_ptr, h->owned);
-  rc |= upload(d->schur, (size_t)d->schur_rows * NT, (uint32_t**)&p.schur, h->owned);
-  rc |= upload((const int4*)d->sf_tasks, (size_t)d->sf_ntasks, (int4**)&p.sf_tasks, h->owned);
-  rc |= upload(d->sf_ptr, (size_t)d->sf_nst + 1, (int**)&p.sf_ptr, h->owned);
-  rc |= upload((const int4*)d->sb_tasks, (size_t)d->sb_ntasks, (int4**)&p.sb_tasks, h->owned);
-  rc |= upload(d->sb_ptr, (size_t)d->sb_nst + 1, (int**)&p.sb_ptr, h->owned);
-  rc |= upload(d->s_pairs, (size_t)d->s_npairs, (uint32_t**)&p.s_pairs, h->owned);
+  rc |= upload((const short*)d->core_src, (size_t)NT * d->slot_stride, (short**)&p.core_src, h->owned);
+  rc |= upload(d->core_row, (size_t)p.mpad, (int**)&p.core_row, h->owned);
+  rc |= upload((const uint2*)d->f_tasks, (size_t)d->n_f_tasks, (uint2**)&p.t.f_tasks, h->owned);
+  rc |= upload(d->f_ptr, (size_t)d->f_nst + 1, (int**)&p.t.f_ptr, h->owned);
+  rc |= upload(d->f_pairs, (size_t)d->n_f_pairs, (uint32_t**)&p.t.f_pairs, h->owned);
+  rc |= upload(d->schur_ptr, (size_t)NT + 1, (int**)&p.t.schur_ptr, h->owned);
+  rc |= upload(d->schur, (size_t)d->n_schur, (uint32_t**)&p.t.schur, h->owned);
+  rc |= upload((const uint2*)d->sf_tasks, (size_t)d->n_sf_tasks, (uint2**)&p.t.sf_tasks, h->owned);
+  rc |= upload(d->sf_ptr, (size_t)d->sf_nst + 1, (int**)&p.t.sf_ptr, h->owned);
+  rc |= upload((const uint2*)d->sb_tasks, (size_t)d->n_sb_tasks, (uint2**)&p.t.sb_tasks, h->owned);
+  rc |= upload(d->sb_ptr, (size_t)d->sb_nst + 1, (int**)&p.t.sb_ptr, h->owned);
+  rc |= upload(d->s_pairs, (size_t)d->n_s_pairs, (uint32_t**)&p.t.s_pairs, h->owned);
   rc |= upload(d->perm, (size_t)d->n, (int**)&p.perm, h->owned);
   if (rc) {
     for (void* q : h->owned) (void)hipFree(q);

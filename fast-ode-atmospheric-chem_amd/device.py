@@ -60,14 +60,23 @@ class DevicePlan:
         plan = build_plan(n, rows, cols, **plan_kw)
         self.plan = plan
         a = plan.arrays
-        arrs = {k: (_u32(v) if k in ("f_pairs", "schur", "s_pairs") else _i32(v)) for k, v in a.items()}
+        unsigned = ("f_tasks", "f_pairs", "schur", "sf_tasks", "sb_tasks", "s_pairs")
+        arrs = {}
+        for k, v in a.items():
+            if k in unsigned:
+                arrs[k] = _u32(v)
+            elif k == "core_src":
+                arrs[k] = np.ascontiguousarray(v, dtype=np.int16)
+            else:
+                arrs[k] = _i32(v)
         d = L.PlanDesc(
             n=plan.n, c=plan.c, m=plan.m, tile=plan.tile, ns=plan.ns, nnz=plan.nnz,
-            f_ntasks=len(a["f_tasks"]) // 4, f_nst=len(a["f_ptr"]) - 1, f_npairs=len(a["f_pairs"]),
-            schur_rows=int(a["schur_ptr"][-1]),
-            sf_ntasks=len(a["sf_tasks"]) // 4, sf_nst=len(a["sf_ptr"]) - 1,
-            sb_ntasks=len(a["sb_tasks"]) // 4, sb_nst=len(a["sb_ptr"]) - 1,
-            s_npairs=len(a["s_pairs"]),
+            slot_stride=plan.slot_stride,
+            n_f_tasks=len(a["f_tasks"]) // 2, f_nst=len(a["f_ptr"]) - 1, n_f_pairs=len(a["f_pairs"]),
+            n_schur=len(a["schur"]),
+            n_sf_tasks=len(a["sf_tasks"]) // 2, sf_nst=len(a["sf_ptr"]) - 1,
+            n_sb_tasks=len(a["sb_tasks"]) // 2, sb_nst=len(a["sb_ptr"]) - 1,
+            n_s_pairs=len(a["s_pairs"]),
             **{k: _addr(v) for k, v in arrs.items()})
         h = ct.c_void_p()
         lu_len = ct.c_int64()

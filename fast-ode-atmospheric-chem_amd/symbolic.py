@@ -18,12 +18,18 @@ pattern fixed here once per mechanism:
    dot-product tasks  A_ij = (A_ij - sum_{p < min(i,j)} L_ip U_pj) [/ U_jj],
    grouped in stages: a task runs once every task it reads has run;
 5. the core's Schur complement update from the sparse pivots,
-   A_ij -= sum_{p < c} L_ip U_pj, is a per-(slot, thread) list of pairs;
+   A_ij -= sum_{p < c} L_ip U_pj, is one list per thread of
+   (slot, L entry, U entry) words in slot order;
 6. the solves use level-scheduled row tasks for the sparse rows and block
    substitution with inverted diagonal blocks for the core.
 
-Everything the kernels read is emitted here as flat integer arrays (see
-``LUPlan.arrays``); the C ABI copies them to the device once.
+Table formats (all little integers, so that they fit in LDS):
+* task  = 2 x uint32: [target | divisor << 16] (divisor 0xFFFF: none) and
+  [first pair | count << 20];
+* pair  = uint32 a << 16 | b  (LDS indices of the two factors);
+* schur = uint32 slot << 24 | a << 12 | b, per thread, in slot order;
+* core_src = int16 [thread][slot_stride]: Jacobian entry, -1 fill, -2 the
+  identity diagonal of a padding row.
 """
 from __future__ import annotations
 
@@ -32,8 +38,10 @@ from dataclasses import dataclass
 import numpy as np
 
 G = 16                 # core thread grid is G x G (256 threads)
+NT = G * G
 TILE_MAX = 10          # largest supported core: G * TILE_MAX = 160
 DENSE_RHO = 0.85       # core = longest trailing block at least this dense
+NO_DIV = 0xFFFF
 
 
 def min_degree_order(n, rows, cols):
@@ -71,16 +79,15 @@ def symbolic_fill(n, rows_new, cols_new):
     return F
 
 
-def choose_split(F, rho=DENSE_RHO, tile_max=TILE_MAX):
+def choose_split(F, rho=DENSE_RHO, m_max=G * TILE_MAX):
     """Core start c: the longest trailing block at least rho dense, no larger
-    than G * tile_max, and at least one pivot."""
+    than m_max, and at least one pivot."""
     n = F.shape[0]
     c = n - 1
     for cand in range(n - 1, -1, -1):
         if F[cand:, cand:].mean() >= rho:
             c = cand
-    c = max(c, n - G * tile_max)
-    return int(c)
+    return int(max(c, n - m_max))
 
 
 def _stage(levels):
@@ -92,6 +99,24 @@ def _stage(levels):
     nst = int(levels.max()) + 1
     counts = np.bincount(levels, minlength=nst)
     return order, np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+
+
+def _pack_tasks(tgt, start, count, div):
+    k = len(tgt)
+    out = np.zeros((k, 2), dtype=np.uint32)
+    if k:
+        tgt, start, count, div = (np.asarray(x, dtype=np.int64) for x in (tgt, start, count, div))
+        if (start >= 1 << 20).any() or (count >= 1 << 12).any():
+            raise ValueError("task pair ranges exceed the packed format")
+        out[:, 0] = tgt | (np.where(div < 0, NO_DIV, div) << 16)
+        out[:, 1] = start | (count << 20)
+    return out.ravel()
+
+
+def _pack_pairs(a, b):
+    a = np.asarray(a, dtype=np.int64)
+    b = np.asarray(b, dtype=np.int64)
+    return ((a << 16) | b).astype(np.uint32)
 
 
 @dataclass
@@ -112,26 +137,32 @@ class LUPlan:
         return G * self.tile
 
     @property
-    def lds_zero(self):
-        return self.ns
+    def slot_stride(self):
+        return -(-self.tile * self.tile // 8) * 8
 
     @property
     def lds_vec(self):
-        return self.ns + 1
+        return self.ns
 
     @property
     def lds_len(self):
-        return self.ns + 1 + self.c + self.m_pad
+        return self.ns + self.c + self.m_pad
 
     @property
     def lu_len(self):
         """Doubles per cell of a stored factorization: sparse entries, the core
         tiles and the inverted diagonal blocks."""
-        return self.ns + self.m_pad * self.m_pad + 2 * self.tile * G * G
+        return self.ns + self.m_pad * self.m_pad + 2 * self.tile * NT
+
+    def table_words(self):
+        """uint32 words of the tables a persistent workgroup keeps in LDS."""
+        a = self.arrays
+        return sum(len(a[k]) for k in ("f_tasks", "f_ptr", "f_pairs", "sf_tasks", "sf_ptr", "sb_tasks",
+                                        "sb_ptr", "s_pairs", "schur_ptr", "schur"))
 
     def flops(self):
-        """Multiply-adds (+ divisions) of one factorization and one solve on
-        the filled pattern, as a no-pivot sparse LU would do them."""
+        """Multiply-adds (+ divisions) of one factorization and the L+U entry
+        count, as a no-pivot sparse LU would do them."""
         n, F = self.n, self.F
         fact = 0
         for k in range(n):
@@ -141,18 +172,21 @@ class LUPlan:
         return fact, int(F.sum())
 
 
-def build_plan(n, rows, cols, tile=None, rho=DENSE_RHO, order=None):
+def build_plan(n, rows, cols, tile=None, rho=DENSE_RHO, order=None, m_max=None):
     """Symbolic analysis of the pattern (rows, cols) (original numbering,
-    diagonal included).  ``order`` overrides the ordering (perm[new] = old)."""
+    diagonal included).  ``order`` overrides the ordering (perm[new] = old);
+    ``m_max`` caps the core size (a smaller register tile)."""
     rows = np.asarray(rows, dtype=np.int64)
     cols = np.asarray(cols, dtype=np.int64)
     nnz = len(rows)
+    if nnz >= 1 << 15:
+        raise ValueError("Jacobian patterns above 32767 entries are not supported")
     perm = min_degree_order(n, rows, cols) if order is None else np.asarray(order, dtype=np.int64)
     inv = np.empty(n, dtype=np.int64)
     inv[perm] = np.arange(n)
     rn, cn = inv[rows], inv[cols]
     F = symbolic_fill(n, rn, cn)
-    c = choose_split(F, rho)
+    c = choose_split(F, rho, G * TILE_MAX if m_max is None else m_max)
     m = n - c
     if tile is None:
         tile = max(1, -(-m // G))
@@ -169,24 +203,22 @@ def build_plan(n, rows, cols, tile=None, rho=DENSE_RHO, order=None):
     si, sj = si[keep], sj[keep]
     ns = len(si)
     sp_index[si, sj] = np.arange(ns)
-    ZERO = ns
-    VEC = ns + 1
-    if ns + 1 + c + m_pad >= 65535:
-        raise ValueError("LDS index space exceeds 16 bits")
+    VEC = ns
+    if ns >= 4096:
+        raise ValueError("more than 4095 sparse entries: the Schur words hold 12-bit indices")
 
     # ---- assembly sources.  Entry value = -J[src] (+ alpha*mass[row] on the
-    # diagonal).  src = -1: structural fill (0); for the core, src = -2 marks
-    # an identity padding diagonal.
+    # diagonal).
     jac_of = -np.ones((n, n), dtype=np.int64)
     jac_of[rn, cn] = np.arange(nnz)
     sp_src = jac_of[si, sj].astype(np.int32)
     sp_diag = np.where(si == sj, perm[si], -1).astype(np.int32)     # old row for mass
     sp_row = perm[si].astype(np.int32)                               # old row (alg-only mask)
 
-    nslot = tile * tile
-    core_src = np.full((nslot, G * G), -1, dtype=np.int32)
-    core_diag = np.full((nslot, G * G), -1, dtype=np.int32)
-    core_row = np.full((nslot, G * G), -1, dtype=np.int32)
+    stride = -(-tile * tile // 8) * 8
+    core_src = np.full((NT, stride), -1, dtype=np.int16)
+    core_row = np.full(m_pad, -1, dtype=np.int32)
+    core_row[:m] = perm[c:]
     for ti in range(tile):
         for tj in range(tile):
             s = ti * tile + tj
@@ -196,24 +228,19 @@ def build_plan(n, rows, cols, tile=None, rho=DENSE_RHO, order=None):
                     ci, cj = ti * G + tr, tj * G + tc
                     if ci >= m or cj >= m:
                         if ci == cj:
-                            core_src[s, t] = -2
+                            core_src[t, s] = -2
                         continue
-                    i, j = c + ci, c + cj
-                    core_src[s, t] = jac_of[i, j]
-                    core_row[s, t] = perm[i]
-                    if i == j:
-                        core_diag[s, t] = perm[i]
+                    core_src[t, s] = jac_of[c + ci, c + cj]
 
     # ---- sparse factorization tasks (dot-product form)
     tgt, pstart, pcount, div, level = [], [], [], [], []
     pairs_a, pairs_b = [], []
-    lev = np.zeros(ns, dtype=np.int64)          # level at which entry is final
-    # process in an order where inputs precede outputs: by min(i, j), then i, j
+    lev = np.zeros(ns, dtype=np.int64)          # stage after which an entry is final
+    # inputs precede outputs: order by min(i, j), then i, j
     order_e = np.lexsort((sj, si, np.minimum(si, sj)))
     for e in order_e:
         i, j = int(si[e]), int(sj[e])
-        lo = min(i, j)
-        ps = [p for p in range(lo) if F[i, p] and F[p, j]]
+        ps = [p for p in range(min(i, j)) if F[i, p] and F[p, j]]
         L = 0
         start = len(pairs_a)
         for p in ps:
@@ -227,23 +254,18 @@ def build_plan(n, rows, cols, tile=None, rho=DENSE_RHO, order=None):
             L = max(L, lev[d] + 1)
         lev[e] = L
         if not ps and d < 0:
-            continue            # nothing to compute: value is the assembled one
+            continue            # the assembled value is final
         tgt.append(e)
         pstart.append(start)
         pcount.append(len(ps))
         div.append(d)
-        level.append(L - 1 if L > 0 else 0)
-    # levels: a task at level L reads entries final at < L.  Shift so that
-    # tasks whose inputs are all assembled values run in stage 0.
+        level.append(L - 1)
     fo, fptr = _stage(level)
-    f_tasks = _pack_tasks(np.asarray(tgt)[fo] if len(tgt) else [], np.asarray(pstart)[fo] if len(tgt) else [],
-                          np.asarray(pcount)[fo] if len(tgt) else [], np.asarray(div)[fo] if len(tgt) else [])
+    f_tasks = _pack_tasks(*(np.asarray(x)[fo] if len(tgt) else [] for x in (tgt, pstart, pcount, div)))
     f_pairs = _pack_pairs(pairs_a, pairs_b)
 
-    # ---- core Schur lists: A_ij -= sum_{p<c} L_ip U_pj, per (slot, thread),
-    # padded per slot to the longest list (pad pairs read the zero entry).
-    lists = {}
-    maxlen = np.zeros(nslot, dtype=np.int64)
+    # ---- core Schur lists: A_ij -= sum_{p<c} L_ip U_pj, per thread in slot order
+    per_thread = [[] for _ in range(NT)]
     for ti in range(tile):
         for tj in range(tile):
             s = ti * tile + tj
@@ -255,15 +277,12 @@ def build_plan(n, rows, cols, tile=None, rho=DENSE_RHO, order=None):
                     i, j = c + ci, c + cj
                     if not F[i, j]:
                         continue
-                    ps = [p for p in range(c) if F[i, p] and F[p, j]]
-                    if ps:
-                        lists[(s, tr * G + tc)] = [(sp_index[i, p], sp_index[p, j]) for p in ps]
-                        maxlen[s] = max(maxlen[s], len(ps))
-    schur_ptr = np.concatenate([[0], np.cumsum(maxlen)]).astype(np.int32)
-    schur = np.full((int(schur_ptr[-1]), G * G), (ZERO << 16) | ZERO, dtype=np.uint32)
-    for (s, t), lst in lists.items():
-        for q, (a, b) in enumerate(lst):
-            schur[schur_ptr[s] + q, t] = (int(a) << 16) | int(b)
+                    for p in range(c):
+                        if F[i, p] and F[p, j]:
+                            per_thread[tr * G + tc].append((s << 24) | (int(sp_index[i, p]) << 12)
+                                                           | int(sp_index[p, j]))
+    schur_ptr = np.concatenate([[0], np.cumsum([len(x) for x in per_thread])]).astype(np.int32)
+    schur = np.asarray([w for x in per_thread for w in x], dtype=np.uint32)
 
     # ---- solve tasks.  Forward (unit L): v_i -= sum_{p<min(i,c)} L_ip v_p.
     ftgt, fst, fcnt, flev = [], [], [], []
@@ -285,8 +304,8 @@ def build_plan(n, rows, cols, tile=None, rho=DENSE_RHO, order=None):
         fcnt.append(len(ps))
         flev.append(L - 1)
     fo2, sfptr = _stage(flev)
-    sf_tasks = _pack_tasks(np.asarray(ftgt)[fo2] if ftgt else [], np.asarray(fst)[fo2] if ftgt else [],
-                           np.asarray(fcnt)[fo2] if ftgt else [], -np.ones(len(ftgt), dtype=np.int64))
+    sf_tasks = _pack_tasks(*(np.asarray(x)[fo2] if ftgt else [] for x in
+                             (ftgt, fst, fcnt, -np.ones(len(ftgt), dtype=np.int64))))
     # Backward over the sparse rows, after the core: v_i = (v_i - sum_{j>i} U_ij v_j) / U_ii.
     btgt, bst, bcnt, bdiv, blev = [], [], [], [], []
     ulev = np.zeros(n, dtype=np.int64)
@@ -305,15 +324,14 @@ def build_plan(n, rows, cols, tile=None, rho=DENSE_RHO, order=None):
         bdiv.append(sp_index[i, i])
         blev.append(L)
     bo, sbptr = _stage(blev)
-    sb_tasks = _pack_tasks(np.asarray(btgt)[bo] if btgt else [], np.asarray(bst)[bo] if btgt else [],
-                           np.asarray(bcnt)[bo] if btgt else [], np.asarray(bdiv)[bo] if btgt else [])
+    sb_tasks = _pack_tasks(*(np.asarray(x)[bo] if btgt else [] for x in (btgt, bst, bcnt, bdiv)))
     s_pairs = _pack_pairs(spa, spb)
 
     arrays = dict(
         sp_src=sp_src, sp_diag=sp_diag, sp_row=sp_row,
-        core_src=core_src.ravel(), core_diag=core_diag.ravel(), core_row=core_row.ravel(),
+        core_src=core_src.ravel(), core_row=core_row,
         f_tasks=f_tasks, f_ptr=fptr.astype(np.int32), f_pairs=f_pairs,
-        schur_ptr=schur_ptr, schur=schur.ravel(),
+        schur_ptr=schur_ptr, schur=schur,
         sf_tasks=sf_tasks, sf_ptr=sfptr.astype(np.int32),
         sb_tasks=sb_tasks, sb_ptr=sbptr.astype(np.int32),
         s_pairs=s_pairs,
@@ -323,76 +341,51 @@ def build_plan(n, rows, cols, tile=None, rho=DENSE_RHO, order=None):
                   nnz_lu=int(F.sum()), arrays=arrays)
 
 
-def _pack_tasks(tgt, start, count, div):
-    """Task = 4 x int32: target LDS index, first pair, pair count, divisor LDS
-    index (-1: none)."""
-    k = len(tgt)
-    out = np.zeros((k, 4), dtype=np.int32)
-    if k:
-        out[:, 0] = tgt
-        out[:, 1] = start
-        out[:, 2] = count
-        out[:, 3] = div
-    return out.ravel()
-
-
-def _pack_pairs(a, b):
-    a = np.asarray(a, dtype=np.int64)
-    b = np.asarray(b, dtype=np.int64)
-    return ((a << 16) | b).astype(np.uint32)
-
-
 # ---------------------------------------------------------------------------
-# A CPU model of exactly what the device does with a plan: used by the tests
-# to check a plan before any kernel runs, never by the product path.
+# A CPU model of exactly what the kernels do with a plan: used by the tests to
+# check a plan before any kernel runs, never by the product path.
 # ---------------------------------------------------------------------------
+
+def _run_tasks(lds, tasks, ptr, pairs):
+    tasks = tasks.reshape(-1, 2).astype(np.int64)
+    pa = (pairs >> 16).astype(np.int64)
+    pb = (pairs & 0xFFFF).astype(np.int64)
+    for s in range(len(ptr) - 1):
+        for t in range(ptr[s], ptr[s + 1]):
+            tg, d = tasks[t, 0] & 0xFFFF, tasks[t, 0] >> 16
+            st, cnt = tasks[t, 1] & 0xFFFFF, tasks[t, 1] >> 20
+            acc = lds[tg]
+            for q in range(st, st + cnt):
+                acc -= lds[pa[q]] * lds[pb[q]]
+            if d != NO_DIV:
+                acc = acc / lds[d]
+            lds[tg] = acc
+
 
 def emulate_factor(plan: LUPlan, A_old: np.ndarray):
     """Factor one dense matrix (original numbering) the way the device does.
-    Returns (lds, core) with lds the sparse entries and core the m_pad^2
-    dense core after no-pivot LU."""
-    n, c, m, mp = plan.n, plan.c, plan.m, plan.m_pad
+    Returns (lds, core): the sparse entries and the m_pad^2 dense core after
+    no-pivot LU."""
+    c, m, mp = plan.c, plan.m, plan.m_pad
     p = plan.perm
     A = A_old[np.ix_(p, p)]
     ar = plan.arrays
     lds = np.zeros(plan.lds_len)
     si, sj = np.nonzero(plan.F)
     keep = np.minimum(si, sj) < c
-    si, sj = si[keep], sj[keep]
-    lds[: plan.ns] = A[si, sj]
-    tasks = ar["f_tasks"].reshape(-1, 4)
-    pa = (ar["f_pairs"] >> 16).astype(np.int64)
-    pb = (ar["f_pairs"] & 0xFFFF).astype(np.int64)
-    ptr = ar["f_ptr"]
-    for s in range(len(ptr) - 1):
-        new = {}
-        for t in range(ptr[s], ptr[s + 1]):
-            tg, st, cnt, d = tasks[t]
-            acc = lds[tg]
-            for q in range(st, st + cnt):
-                acc -= lds[pa[q]] * lds[pb[q]]
-            if d >= 0:
-                acc = acc / lds[d]
-            new[tg] = acc
-        for tg, v in new.items():
-            lds[tg] = v
+    lds[: plan.ns] = A[si[keep], sj[keep]]
+    _run_tasks(lds, ar["f_tasks"], ar["f_ptr"], ar["f_pairs"])
     core = np.eye(mp)
     core[:m, :m] = A[c:, c:] * plan.F[c:, c:]
-    # Schur lists
     sp = ar["schur_ptr"]
-    sch = ar["schur"].reshape(-1, G * G)
     tile = plan.tile
-    for ti in range(tile):
-        for tj in range(tile):
-            s = ti * tile + tj
-            for t in range(G * G):
-                tr, tc = divmod(t, G)
-                ci, cj = ti * G + tr, tj * G + tc
-                acc = core[ci, cj]
-                for q in range(sp[s], sp[s + 1]):
-                    w = int(sch[q, t])
-                    acc -= lds[w >> 16] * lds[w & 0xFFFF]
-                core[ci, cj] = acc
+    for t in range(NT):
+        tr, tc = divmod(t, G)
+        for w in ar["schur"][sp[t]: sp[t + 1]]:
+            w = int(w)
+            s = w >> 24
+            ti, tj = divmod(s, tile)
+            core[ti * G + tr, tj * G + tc] -= lds[(w >> 12) & 0xFFF] * lds[w & 0xFFF]
     for k in range(mp):
         core[k + 1:, k] /= core[k, k]
         core[k + 1:, k + 1:] -= np.outer(core[k + 1:, k], core[k, k + 1:])
@@ -400,39 +393,19 @@ def emulate_factor(plan: LUPlan, A_old: np.ndarray):
 
 
 def emulate_solve(plan: LUPlan, lds, core, b_old):
-    n, c, m, mp = plan.n, plan.c, plan.m, plan.m_pad
+    n, c, mp = plan.n, plan.c, plan.m_pad
     p = plan.perm
     lds = lds.copy()
     V = plan.lds_vec
     lds[V: V + n] = b_old[p]
     lds[V + n:] = 0.0
     ar = plan.arrays
-    pa = (ar["s_pairs"] >> 16).astype(np.int64)
-    pb = (ar["s_pairs"] & 0xFFFF).astype(np.int64)
-
-    def run(tasks, ptr):
-        tasks = tasks.reshape(-1, 4)
-        for s in range(len(ptr) - 1):
-            new = {}
-            for t in range(ptr[s], ptr[s + 1]):
-                tg, st, cnt, d = tasks[t]
-                acc = lds[tg]
-                for q in range(st, st + cnt):
-                    acc -= lds[pa[q]] * lds[pb[q]]
-                if d >= 0:
-                    acc = acc / lds[d]
-                new[tg] = acc
-            for tg, v in new.items():
-                lds[tg] = v
-
-    run(ar["sf_tasks"], ar["sf_ptr"])
+    _run_tasks(lds, ar["sf_tasks"], ar["sf_ptr"], ar["s_pairs"])
     y = lds[V + c: V + c + mp].copy()
-    Lc = np.tril(core, -1) + np.eye(mp)
-    Uc = np.triu(core)
-    y = np.linalg.solve(Lc, y)
-    y = np.linalg.solve(Uc, y)
+    y = np.linalg.solve(np.tril(core, -1) + np.eye(mp), y)
+    y = np.linalg.solve(np.triu(core), y)
     lds[V + c: V + c + mp] = y
-    run(ar["sb_tasks"], ar["sb_ptr"])
+    _run_tasks(lds, ar["sb_tasks"], ar["sb_ptr"], ar["s_pairs"])
     x = np.empty(n)
     x[p] = lds[V: V + n]
     return x

@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 import torch
 
-from golden_util import COUNT_NAMES, GOLDEN, load, workload
+from golden_util import GOLDEN, load, state_error, workload
 
 pytestmark = pytest.mark.gpu
 
@@ -50,12 +50,9 @@ def replay_batched(name, graph):
             s = list(g["snap_interval"]).index(it + 1)
             ref = g["snaps"][s]
             Y = run.Y.cpu().numpy()
-            big = np.abs(ref) > 1e-17
-            rel = np.where(big, np.abs(Y - ref) / np.maximum(np.abs(ref), 1e-300), 0.0)
-            at = np.unravel_index(np.argmax(rel), rel.shape)
-            print(f"{name} interval {it + 1}: worst rel {rel.max():.3e} at value {ref[at]:.3e} cell {at[0]} "
-                  f"species {at[1]}")
-            worst = max(worst, float(rel.max()))
+            scaled, rel = state_error(Y, ref, wl.atol)
+            print(f"{name} interval {it + 1}: scaled error {scaled:.3e}, relative above the floor {rel:.3e}")
+            worst = max(worst, scaled)
     mem = rec
     assert np.allclose(mem["h_next"], g["memory"][:, 0], rtol=1e-12)
     return worst, run.Y.cpu().numpy()
@@ -69,20 +66,20 @@ def fa_status(code):
 @pytest.mark.parametrize("graph", [False, True])
 def test_batched_matches_golden_mech20(graph):
     worst, _ = replay_batched("mech20_dense", graph)
-    assert worst < 1e-9
+    assert worst <= 1.0
 
 
 def test_batched_matches_golden_mech200_diurnal():
     worst, _ = replay_batched("mech200_diurnal", True)
     print("batched mech200 worst rel", worst)
-    assert worst < 1e-9
+    assert worst <= 1.0
 
 
 def test_batched_matches_golden_mech200_uneven():
     if not (GOLDEN / "mech200_uneven.npz").exists():
         pytest.skip("golden vectors not generated")
     worst, _ = replay_batched("mech200_uneven", True)
-    assert worst < 1e-9
+    assert worst <= 1.0
 
 
 def test_batched_equals_plain_bitwise_and_graph_replays_match_eager():

@@ -59,18 +59,24 @@ def test_plan_fill_is_a_superset_and_levels_are_causal():
     assert plan.F[inv[mech.rows], inv[mech.cols]].all()
     # every factor task's pairs read entries final before its stage
     a = plan.arrays
-    tasks = a["f_tasks"].reshape(-1, 4)
+    tasks = a["f_tasks"].reshape(-1, 2).astype(np.int64)
     ptr = a["f_ptr"]
     done_stage = {}
     for s in range(len(ptr) - 1):
         for t in range(ptr[s], ptr[s + 1]):
-            done_stage[int(tasks[t, 0])] = s
+            done_stage[int(tasks[t, 0] & 0xFFFF)] = s
     for s in range(len(ptr) - 1):
         for t in range(ptr[s], ptr[s + 1]):
-            for q in range(tasks[t, 1], tasks[t, 1] + tasks[t, 2]):
+            start, count = tasks[t, 1] & 0xFFFFF, tasks[t, 1] >> 20
+            for q in range(start, start + count):
                 w = int(a["f_pairs"][q])
                 for src in (w >> 16, w & 0xFFFF):
                     assert done_stage.get(src, -1) < s
+    # Schur words are in slot order within each thread
+    sp = a["schur_ptr"]
+    for t in range(256):
+        slots = a["schur"][sp[t]:sp[t + 1]] >> 24
+        assert (np.diff(slots.astype(np.int64)) >= 0).all()
 
 
 def test_mechanism_tables_match_the_oracle():
