@@ -65,15 +65,13 @@ class MechDesc(ct.Structure):
 
 class PlanDesc(ct.Structure):
     _fields_ = [("n", ct.c_int32), ("c", ct.c_int32), ("m", ct.c_int32), ("tile", ct.c_int32),
-                ("ns", ct.c_int32), ("nnz", ct.c_int32), ("slot_stride", ct.c_int32),
+                ("ns", ct.c_int32), ("nsch", ct.c_int32), ("nnz", ct.c_int32), ("slot_stride", ct.c_int32),
                 ("n_f_tasks", ct.c_int32), ("f_nst", ct.c_int32), ("n_f_pairs", ct.c_int32),
-                ("n_schur", ct.c_int32),
                 ("n_sf_tasks", ct.c_int32), ("sf_nst", ct.c_int32), ("n_sb_tasks", ct.c_int32),
                 ("sb_nst", ct.c_int32), ("n_s_pairs", ct.c_int32),
                 ("sp_src", ct.c_void_p), ("sp_diag", ct.c_void_p), ("sp_row", ct.c_void_p),
-                ("core_src", ct.c_void_p), ("core_row", ct.c_void_p),
+                ("core_src", ct.c_void_p), ("core_sch", ct.c_void_p), ("core_row", ct.c_void_p),
                 ("f_tasks", ct.c_void_p), ("f_ptr", ct.c_void_p), ("f_pairs", ct.c_void_p),
-                ("schur_ptr", ct.c_void_p), ("schur", ct.c_void_p),
                 ("sf_tasks", ct.c_void_p), ("sf_ptr", ct.c_void_p),
                 ("sb_tasks", ct.c_void_p), ("sb_ptr", ct.c_void_p),
                 ("s_pairs", ct.c_void_p), ("perm", ct.c_void_p)]

@@ -53,11 +53,12 @@ __host__ __device__ inline int cell_region_doubles(int n, int R, int S) {
   return 4 * n + S * n + R + 4 + 2;
 }
 
-Layout make_layout(const PlanDev& P, const MechDev& M, int S) {
+// stage: copy the task tables into LDS (when they fit beside the cell's data).
+Layout make_layout(const PlanDev& P, const MechDev& M, int S, bool stage) {
   Layout l;
   const int words = table_words(P.n_f_tasks, P.f_nst, P.n_f_pairs, P.n_sf_tasks, P.sf_nst,
-                                P.n_sb_tasks, P.sb_nst, P.n_s_pairs, P.n_schur);
-  l.tables = (words + 1) / 2;
+                                P.n_sb_tasks, P.sb_nst, P.n_s_pairs);
+  l.tables = stage ? (words + 1) / 2 : 0;
   l.engine = l.tables;
   int end = l.engine + engine_lds_doubles(P.lds_len, P.mpad, P.tile);
   // Rates (R) and rate partials (R*P) are needed between the solves, when
@@ -79,6 +80,16 @@ Layout make_layout(const PlanDev& P, const MechDev& M, int S) {
   l.cell = end;
   l.total = l.cell + cell_region_doubles(M.n, M.R, S);
   return l;
+}
+
+// Tables in LDS when everything fits in one workgroup's LDS, else in memory.
+Layout choose_layout(const PlanDev& P, const MechDev& M, int S) {
+  int dev = 0, maxlds = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+  const Layout staged = make_layout(P, M, S, true);
+  if ((size_t)staged.total * sizeof(double) + sizeof(fode_group_ctrl) <= (size_t)maxlds) return staged;
+  return make_layout(P, M, S, false);
 }
 
 struct CellLds {
@@ -135,7 +146,7 @@ __device__ double cell_attempt(const BatchArgs& A, const Tables& T, const Engine
   assemble<TILE>(P, JLds{V.jv}, g.alpha, A.b.mass, false, L.lds, a);
   factor_cell<TILE>(P, T, L, a);
   // Stages (fastode/solvers.py:54-86).
-  double* vec = L.lds + P.ns;
+  double* vec = L.lds + P.vec;
   const int S = p.stages;
   const bool dae = p.constrained != 0;
   const double* forcing_src = V.f0;
@@ -194,7 +205,7 @@ template <int TILE>
 __global__ __launch_bounds__(NT) void k_batched(BatchArgs A) {
   extern __shared__ double smem[];
   const int n = A.M.n, S = A.p.stages;
-  const Tables T = stage_tables(A.P, reinterpret_cast<uint32_t*>(smem));
+  const Tables T = A.lay.tables > 0 ? stage_tables(A.P, reinterpret_cast<uint32_t*>(smem)) : A.P.t;
   const EngineLds L = carve(smem + A.lay.engine, A.P);
   CellLds V;
   V.rd = smem + A.lay.rd;
@@ -301,7 +312,7 @@ int fode_batched_lds_bytes(void* plan, void* mech, int stages, int64_t* bytes) {
   if (!plan || !mech || !bytes || stages < 1 || stages > 4) return bfail("fode_batched_lds_bytes: bad argument");
   const PlanDev& P = *static_cast<const PlanDev*>(fode_internal_plan(plan));
   const MechDev& M = *static_cast<const MechDev*>(fode_internal_mech(mech));
-  *bytes = (int64_t)make_layout(P, M, stages).total * (int64_t)sizeof(double);
+  *bytes = (int64_t)choose_layout(P, M, stages).total * (int64_t)sizeof(double);
   return 0;
 }
 
@@ -372,7 +383,7 @@ int fode_batched_interval(void* plan, void* mech, const fode_ctrl_params* p, con
   if (A.P.n != A.M.n || A.P.nnz != A.M.nnz) return bfail("fode_batched_interval: plan and mechanism differ");
   A.p = *p;
   A.b = *b;
-  A.lay = make_layout(A.P, A.M, p->stages);
+  A.lay = choose_layout(A.P, A.M, p->stages);
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   switch (A.P.tile) {
 #define X(T) case T: return launch_batched<T>(A, blocks_per_cu, s);

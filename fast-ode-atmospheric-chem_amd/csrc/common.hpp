@@ -21,19 +21,18 @@ struct Tables {
   const uint2* sf_tasks; const int* sf_ptr;
   const uint2* sb_tasks; const int* sb_ptr;
   const uint32_t* s_pairs;
-  const int* schur_ptr;        // NT + 1
-  const uint32_t* schur;       // slot << 24 | a << 12 | b
 };
 
 struct PlanDev {
-  int n, c, m, tile, ns, nnz;
-  int lds_len, lu_len, mpad, slot_stride;
+  int n, c, m, tile, ns, nsch, nnz;
+  int lds_len, lu_len, mpad, slot_stride, vec;   // vec = ns + nsch
   int f_nst, sf_nst, sb_nst;
-  // sizes (uint32 words) of the table arrays, for staging them into LDS
-  int n_f_tasks, n_f_pairs, n_sf_tasks, n_sb_tasks, n_s_pairs, n_schur;
+  // sizes (tasks, pairs) of the tables, for staging them into LDS
+  int n_f_tasks, n_f_pairs, n_sf_tasks, n_sb_tasks, n_s_pairs;
   Tables t;                    // global-memory tables
   const int* sp_src; const int* sp_diag; const int* sp_row;
-  const short* core_src;       // [NT][slot_stride]
+  const short* core_src;       // [NT][slot_stride] Jacobian entry, -1 fill, -2 identity
+  const short* core_sch;       // [NT][slot_stride] Schur sum index, -1 none
   const int* core_row;         // [mpad] old row of core row, -1 padding
   const int* perm;             // new -> old
 };

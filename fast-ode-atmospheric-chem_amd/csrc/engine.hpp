@@ -104,20 +104,17 @@ __device__ inline Tables stage_tables(const PlanDev& P, uint32_t* w) {
   t.sb_tasks = reinterpret_cast<const uint2*>(take(P.t.sb_tasks, 2 * P.n_sb_tasks));
   t.sb_ptr = reinterpret_cast<const int*>(take(P.t.sb_ptr, P.sb_nst + 1));
   t.s_pairs = take(P.t.s_pairs, P.n_s_pairs);
-  t.schur_ptr = reinterpret_cast<const int*>(take(P.t.schur_ptr, NT + 1));
-  t.schur = take(P.t.schur, P.n_schur);
   __syncthreads();
   return t;
 }
 
 __host__ __device__ inline int table_words(int n_f_tasks, int f_nst, int n_f_pairs, int n_sf_tasks,
-                                           int sf_nst, int n_sb_tasks, int sb_nst, int n_s_pairs,
-                                           int n_schur) {
+                                           int sf_nst, int n_sb_tasks, int sb_nst, int n_s_pairs) {
   int w = 2 * n_f_tasks + f_nst + 1 + n_f_pairs;
   w += w & 1;
   w += 2 * n_sf_tasks + sf_nst + 1;
   w += w & 1;
-  w += 2 * n_sb_tasks + sb_nst + 1 + n_s_pairs + NT + 1 + n_schur;
+  w += 2 * n_sb_tasks + sb_nst + 1 + n_s_pairs;
   return w;
 }
 
@@ -147,6 +144,7 @@ __device__ inline void assemble(const PlanDev& P, const JS& jv, double alpha, co
     lds[s] = stage_entry(jv, P.sp_src[s], mass[P.sp_row[s]] == 0.0, d >= 0, alpha,
                          d >= 0 ? mass[d] : 0.0, alg_only);
   }
+  for (int s = tid; s < P.nsch; s += NT) lds[P.ns + s] = 0.0;   // Schur sums start at zero
   // the row of each of this thread's core rows: its mass (alg-only mask, diagonal)
   double rmass[TILE];
 #pragma unroll
@@ -166,26 +164,17 @@ __device__ inline void assemble(const PlanDev& P, const JS& jv, double alpha, co
     }
 }
 
-// Schur update of the core from the sparse pivots: this thread's words, in
-// slot order.
+// Schur update of the core from the sparse pivots: the sums the factor tasks
+// left in LDS are added to the thread's entries.
 template <int TILE>
-__device__ inline void schur(const Tables& T, const double* lds, double (&a)[TILE][TILE]) {
-  const int tid = threadIdx.x;
-  int k = T.schur_ptr[tid];
-  const int end = T.schur_ptr[tid + 1];
-  uint32_t w = k < end ? T.schur[k] : 0xFFFFFFFFu;
+__device__ inline void schur(const PlanDev& P, const double* lds, double (&a)[TILE][TILE]) {
+  const short* sch = P.core_sch + (size_t)threadIdx.x * P.slot_stride;
 #pragma unroll
   for (int ti = 0; ti < TILE; ++ti)
 #pragma unroll
     for (int tj = 0; tj < TILE; ++tj) {
-      const uint32_t s = (uint32_t)(ti * TILE + tj);
-      double acc = a[ti][tj];
-      while ((w >> 24) == s) {
-        acc = fma(-lds[(w >> 12) & 0xFFFu], lds[w & 0xFFFu], acc);
-        ++k;
-        w = k < end ? T.schur[k] : 0xFFFFFFFFu;
-      }
-      a[ti][tj] = acc;
+      const int k = sch[ti * TILE + tj];
+      if (k >= 0) a[ti][tj] = a[ti][tj] + lds[P.ns + k];
     }
 }
 
@@ -346,7 +335,7 @@ __device__ inline void factor_cell(const PlanDev& P, const Tables& T, const Engi
                                    double (&a)[TILE][TILE]) {
   __syncthreads();
   run_tasks(T.f_tasks, T.f_ptr, P.f_nst, T.f_pairs, L.lds);
-  schur<TILE>(T, L.lds, a);
+  schur<TILE>(P, L.lds, a);
   core_factor<TILE>(a, L);
   diag_inverses<TILE>(a, L);
 }
@@ -356,7 +345,7 @@ __device__ inline void factor_cell(const PlanDev& P, const Tables& T, const Engi
 template <int TILE>
 __device__ inline void solve_cell(const PlanDev& P, const Tables& T, const EngineLds& L,
                                   const double (&a)[TILE][TILE]) {
-  double* vec = L.lds + P.ns;
+  double* vec = L.lds + P.vec;
   __syncthreads();
   run_tasks(T.sf_tasks, T.sf_ptr, P.sf_nst, T.s_pairs, L.lds);
   core_solve<TILE>(a, L, vec + P.c);

@@ -137,7 +137,7 @@ __global__ __launch_bounds__(NT) void k_solve(PlanDev P, const double* __restric
     for (int tj = 0; tj < TILE; ++tj) a[ti][tj] = core[(ti * TILE + tj) * NT + tid];
   const double* dd = core + TILE * TILE * NT;
   for (int i = tid; i < 2 * TILE * NT; i += NT) L.dL[i] = dd[i];
-  double* vec = L.lds + P.ns;
+  double* vec = L.lds + P.vec;
   const double* b = B + (size_t)cell * P.n;
   for (int i = tid; i < P.c + P.mpad; i += NT) vec[i] = (i < P.n) ? b[P.perm[i]] : 0.0;
   solve_cell<TILE>(P, P.t, L, a);
@@ -362,25 +362,25 @@ int fode_plan_create(const fode_plan_desc* d, void** out, int64_t* lu_len) {
   if (d->slot_stride < d->tile * d->tile || d->slot_stride % 8) return fail("fode_plan_create: bad slot stride");
   auto* h = new PlanHandle();
   PlanDev& p = h->d;
-  p.n = d->n; p.c = d->c; p.m = d->m; p.tile = d->tile; p.ns = d->ns; p.nnz = d->nnz;
+  p.n = d->n; p.c = d->c; p.m = d->m; p.tile = d->tile; p.ns = d->ns; p.nsch = d->nsch; p.nnz = d->nnz;
   p.mpad = G * d->tile;
   p.slot_stride = d->slot_stride;
-  p.lds_len = d->ns + d->c + p.mpad;
+  p.vec = d->ns + d->nsch;
+  p.lds_len = p.vec + d->c + p.mpad;
   p.lu_len = d->ns + p.mpad * p.mpad + 2 * d->tile * NT;
   p.f_nst = d->f_nst; p.sf_nst = d->sf_nst; p.sb_nst = d->sb_nst;
   p.n_f_tasks = d->n_f_tasks; p.n_f_pairs = d->n_f_pairs; p.n_sf_tasks = d->n_sf_tasks;
-  p.n_sb_tasks = d->n_sb_tasks; p.n_s_pairs = d->n_s_pairs; p.n_schur = d->n_schur;
+  p.n_sb_tasks = d->n_sb_tasks; p.n_s_pairs = d->n_s_pairs;
   int rc = 0;
   rc |= upload(d->sp_src, (size_t)d->ns, (int**)&p.sp_src, h->owned);
   rc |= upload(d->sp_diag, (size_t)d->ns, (int**)&p.sp_diag, h->owned);
   rc |= upload(d->sp_row, (size_t)d->ns, (int**)&p.sp_row, h->owned);
   rc |= upload((const short*)d->core_src, (size_t)NT * d->slot_stride, (short**)&p.core_src, h->owned);
+  rc |= upload((const short*)d->core_sch, (size_t)NT * d->slot_stride, (short**)&p.core_sch, h->owned);
   rc |= upload(d->core_row, (size_t)p.mpad, (int**)&p.core_row, h->owned);
   rc |= upload((const uint2*)d->f_tasks, (size_t)d->n_f_tasks, (uint2**)&p.t.f_tasks, h->owned);
   rc |= upload(d->f_ptr, (size_t)d->f_nst + 1, (int**)&p.t.f_ptr, h->owned);
   rc |= upload(d->f_pairs, (size_t)d->n_f_pairs, (uint32_t**)&p.t.f_pairs, h->owned);
-  rc |= upload(d->schur_ptr, (size_t)NT + 1, (int**)&p.t.schur_ptr, h->owned);
-  rc |= upload(d->schur, (size_t)d->n_schur, (uint32_t**)&p.t.schur, h->owned);
   rc |= upload((const uint2*)d->sf_tasks, (size_t)d->n_sf_tasks, (uint2**)&p.t.sf_tasks, h->owned);
   rc |= upload(d->sf_ptr, (size_t)d->sf_nst + 1, (int**)&p.t.sf_ptr, h->owned);
   rc |= upload((const uint2*)d->sb_tasks, (size_t)d->n_sb_tasks, (uint2**)&p.t.sb_tasks, h->owned);

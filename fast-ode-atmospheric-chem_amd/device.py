@@ -60,20 +60,19 @@ class DevicePlan:
         plan = build_plan(n, rows, cols, **plan_kw)
         self.plan = plan
         a = plan.arrays
-        unsigned = ("f_tasks", "f_pairs", "schur", "sf_tasks", "sb_tasks", "s_pairs")
+        unsigned = ("f_tasks", "f_pairs", "sf_tasks", "sb_tasks", "s_pairs")
         arrs = {}
         for k, v in a.items():
             if k in unsigned:
                 arrs[k] = _u32(v)
-            elif k == "core_src":
+            elif k in ("core_src", "core_sch"):
                 arrs[k] = np.ascontiguousarray(v, dtype=np.int16)
             else:
                 arrs[k] = _i32(v)
         d = L.PlanDesc(
-            n=plan.n, c=plan.c, m=plan.m, tile=plan.tile, ns=plan.ns, nnz=plan.nnz,
+            n=plan.n, c=plan.c, m=plan.m, tile=plan.tile, ns=plan.ns, nsch=plan.nsch, nnz=plan.nnz,
             slot_stride=plan.slot_stride,
             n_f_tasks=len(a["f_tasks"]) // 2, f_nst=len(a["f_ptr"]) - 1, n_f_pairs=len(a["f_pairs"]),
-            n_schur=len(a["schur"]),
             n_sf_tasks=len(a["sf_tasks"]) // 2, sf_nst=len(a["sf_ptr"]) - 1,
             n_sb_tasks=len(a["sb_tasks"]) // 2, sb_nst=len(a["sb_ptr"]) - 1,
             n_s_pairs=len(a["s_pairs"]),

@@ -18,18 +18,22 @@ pattern fixed here once per mechanism:
    dot-product tasks  A_ij = (A_ij - sum_{p < min(i,j)} L_ip U_pj) [/ U_jj],
    grouped in stages: a task runs once every task it reads has run;
 5. the core's Schur complement update from the sparse pivots,
-   A_ij -= sum_{p < c} L_ip U_pj, is one list per thread of
-   (slot, L entry, U entry) words in slot order;
+   S_ij = -sum_{p < c} L_ip U_pj, is one more dot-product task per core entry
+   that has one, writing S_ij to its own LDS slot; each thread then adds the
+   sums of its entries to its register tile;
 6. the solves use level-scheduled row tasks for the sparse rows and block
    substitution with inverted diagonal blocks for the core.
+
+LDS index space of one cell: [0, ns) sparse entries, [ns, ns + nsch) Schur
+sums, then the solve vector (c + m_pad entries).
 
 Table formats (all little integers, so that they fit in LDS):
 * task  = 2 x uint32: [target | divisor << 16] (divisor 0xFFFF: none) and
   [first pair | count << 20];
 * pair  = uint32 a << 16 | b  (LDS indices of the two factors);
-* schur = uint32 slot << 24 | a << 12 | b, per thread, in slot order;
 * core_src = int16 [thread][slot_stride]: Jacobian entry, -1 fill, -2 the
-  identity diagonal of a padding row.
+  identity diagonal of a padding row;
+* core_sch = int16 [thread][slot_stride]: Schur sum of the entry, -1 none.
 """
 from __future__ import annotations
 
@@ -129,6 +133,7 @@ class LUPlan:
     tile: int                # core tile per thread
     F: np.ndarray            # filled pattern (new ordering)
     ns: int                  # sparse LU entries (stored in LDS)
+    nsch: int                # core entries with a Schur sum
     nnz_lu: int              # all L+U entries of the filled pattern
     arrays: dict             # flat arrays uploaded to the device
 
@@ -142,11 +147,11 @@ class LUPlan:
 
     @property
     def lds_vec(self):
-        return self.ns
+        return self.ns + self.nsch
 
     @property
     def lds_len(self):
-        return self.ns + self.c + self.m_pad
+        return self.ns + self.nsch + self.c + self.m_pad
 
     @property
     def lu_len(self):
@@ -158,7 +163,7 @@ class LUPlan:
         """uint32 words of the tables a persistent workgroup keeps in LDS."""
         a = self.arrays
         return sum(len(a[k]) for k in ("f_tasks", "f_ptr", "f_pairs", "sf_tasks", "sf_ptr", "sb_tasks",
-                                        "sb_ptr", "s_pairs", "schur_ptr", "schur"))
+                                        "sb_ptr", "s_pairs"))
 
     def flops(self):
         """Multiply-adds (+ divisions) of one factorization and the L+U entry
@@ -203,9 +208,23 @@ def build_plan(n, rows, cols, tile=None, rho=DENSE_RHO, order=None, m_max=None):
     si, sj = si[keep], sj[keep]
     ns = len(si)
     sp_index[si, sj] = np.arange(ns)
-    VEC = ns
-    if ns >= 4096:
-        raise ValueError("more than 4095 sparse entries: the Schur words hold 12-bit indices")
+
+    # ---- core entries updated by sparse pivots get an LDS Schur sum each
+    sch_entries = []                            # (ci, cj, [(L_ip, U_pj) ...])
+    for ci in range(m):
+        i = c + ci
+        prow = [p for p in range(c) if F[i, p]]
+        if not prow:
+            continue
+        for cj in range(m):
+            j = c + cj
+            ps = [p for p in prow if F[p, j]]
+            if ps:
+                sch_entries.append((ci, cj, [(int(sp_index[i, p]), int(sp_index[p, j])) for p in ps]))
+    nsch = len(sch_entries)
+    VEC = ns + nsch
+    if VEC + c + m_pad >= NO_DIV:
+        raise ValueError("the LDS index space of one cell exceeds 16 bits")
 
     # ---- assembly sources.  Entry value = -J[src] (+ alpha*mass[row] on the
     # diagonal).
@@ -231,6 +250,11 @@ def build_plan(n, rows, cols, tile=None, rho=DENSE_RHO, order=None, m_max=None):
                             core_src[t, s] = -2
                         continue
                     core_src[t, s] = jac_of[c + ci, c + cj]
+    core_sch = np.full((NT, stride), -1, dtype=np.int16)
+    for k, (ci, cj, _) in enumerate(sch_entries):
+        ti, tr = divmod(ci, G)
+        tj, tc = divmod(cj, G)
+        core_sch[tr * G + tc, ti * tile + tj] = k
 
     # ---- sparse factorization tasks (dot-product form)
     tgt, pstart, pcount, div, level = [], [], [], [], []
@@ -260,29 +284,22 @@ def build_plan(n, rows, cols, tile=None, rho=DENSE_RHO, order=None, m_max=None):
         pcount.append(len(ps))
         div.append(d)
         level.append(L - 1)
+    # Schur sums S = 0 - sum_p L_ip U_pj (the slot starts at zero)
+    for k, (_, _, prs) in enumerate(sch_entries):
+        start = len(pairs_a)
+        L = 0
+        for a, b in prs:
+            pairs_a.append(a)
+            pairs_b.append(b)
+            L = max(L, lev[a] + 1, lev[b] + 1)
+        tgt.append(ns + k)
+        pstart.append(start)
+        pcount.append(len(prs))
+        div.append(-1)
+        level.append(L - 1)
     fo, fptr = _stage(level)
     f_tasks = _pack_tasks(*(np.asarray(x)[fo] if len(tgt) else [] for x in (tgt, pstart, pcount, div)))
     f_pairs = _pack_pairs(pairs_a, pairs_b)
-
-    # ---- core Schur lists: A_ij -= sum_{p<c} L_ip U_pj, per thread in slot order
-    per_thread = [[] for _ in range(NT)]
-    for ti in range(tile):
-        for tj in range(tile):
-            s = ti * tile + tj
-            for tr in range(G):
-                for tc in range(G):
-                    ci, cj = ti * G + tr, tj * G + tc
-                    if ci >= m or cj >= m:
-                        continue
-                    i, j = c + ci, c + cj
-                    if not F[i, j]:
-                        continue
-                    for p in range(c):
-                        if F[i, p] and F[p, j]:
-                            per_thread[tr * G + tc].append((s << 24) | (int(sp_index[i, p]) << 12)
-                                                           | int(sp_index[p, j]))
-    schur_ptr = np.concatenate([[0], np.cumsum([len(x) for x in per_thread])]).astype(np.int32)
-    schur = np.asarray([w for x in per_thread for w in x], dtype=np.uint32)
 
     # ---- solve tasks.  Forward (unit L): v_i -= sum_{p<min(i,c)} L_ip v_p.
     ftgt, fst, fcnt, flev = [], [], [], []
@@ -329,15 +346,14 @@ def build_plan(n, rows, cols, tile=None, rho=DENSE_RHO, order=None, m_max=None):
 
     arrays = dict(
         sp_src=sp_src, sp_diag=sp_diag, sp_row=sp_row,
-        core_src=core_src.ravel(), core_row=core_row,
+        core_src=core_src.ravel(), core_sch=core_sch.ravel(), core_row=core_row,
         f_tasks=f_tasks, f_ptr=fptr.astype(np.int32), f_pairs=f_pairs,
-        schur_ptr=schur_ptr, schur=schur,
         sf_tasks=sf_tasks, sf_ptr=sfptr.astype(np.int32),
         sb_tasks=sb_tasks, sb_ptr=sbptr.astype(np.int32),
         s_pairs=s_pairs,
         perm=perm.astype(np.int32),
     )
-    return LUPlan(n=n, nnz=nnz, perm=perm, c=c, m=m, tile=tile, F=F, ns=ns,
+    return LUPlan(n=n, nnz=nnz, perm=perm, c=c, m=m, tile=tile, F=F, ns=ns, nsch=nsch,
                   nnz_lu=int(F.sum()), arrays=arrays)
 
 
@@ -377,15 +393,15 @@ def emulate_factor(plan: LUPlan, A_old: np.ndarray):
     _run_tasks(lds, ar["f_tasks"], ar["f_ptr"], ar["f_pairs"])
     core = np.eye(mp)
     core[:m, :m] = A[c:, c:] * plan.F[c:, c:]
-    sp = ar["schur_ptr"]
+    sch = ar["core_sch"].reshape(NT, plan.slot_stride)
     tile = plan.tile
     for t in range(NT):
         tr, tc = divmod(t, G)
-        for w in ar["schur"][sp[t]: sp[t + 1]]:
-            w = int(w)
-            s = w >> 24
-            ti, tj = divmod(s, tile)
-            core[ti * G + tr, tj * G + tc] -= lds[(w >> 12) & 0xFFF] * lds[w & 0xFFF]
+        for s in range(tile * tile):
+            k = int(sch[t, s])
+            if k >= 0:
+                ti, tj = divmod(s, tile)
+                core[ti * G + tr, tj * G + tc] += lds[plan.ns + k]
     for k in range(mp):
         core[k + 1:, k] /= core[k, k]
         core[k + 1:, k + 1:] -= np.outer(core[k + 1:, k], core[k, k + 1:])

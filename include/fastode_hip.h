@@ -82,16 +82,14 @@ typedef struct fode_mech_desc {
 /* Symbolic LU plan (see the package's symbolic.py for the table formats).
  * Tasks are 2 x uint32 each; counts are in tasks, pairs or words. */
 typedef struct fode_plan_desc {
-  int32_t n, c, m, tile, ns, nnz, slot_stride;
+  int32_t n, c, m, tile, ns, nsch, nnz, slot_stride;
   int32_t n_f_tasks, f_nst, n_f_pairs;
-  int32_t n_schur;
   int32_t n_sf_tasks, sf_nst, n_sb_tasks, sb_nst, n_s_pairs;
   const int32_t* sp_src; const int32_t* sp_diag; const int32_t* sp_row;     /* ns */
   const int16_t* core_src;   /* 256 * slot_stride */
+  const int16_t* core_sch;   /* 256 * slot_stride */
   const int32_t* core_row;   /* 16 * tile */
   const uint32_t* f_tasks; const int32_t* f_ptr; const uint32_t* f_pairs;
-  const int32_t* schur_ptr;  /* 257 */
-  const uint32_t* schur;
   const uint32_t* sf_tasks; const int32_t* sf_ptr;
   const uint32_t* sb_tasks; const int32_t* sb_ptr;
   const uint32_t* s_pairs;

@@ -72,11 +72,10 @@ def test_plan_fill_is_a_superset_and_levels_are_causal():
                 w = int(a["f_pairs"][q])
                 for src in (w >> 16, w & 0xFFFF):
                     assert done_stage.get(src, -1) < s
-    # Schur words are in slot order within each thread
-    sp = a["schur_ptr"]
-    for t in range(256):
-        slots = a["schur"][sp[t]:sp[t + 1]] >> 24
-        assert (np.diff(slots.astype(np.int64)) >= 0).all()
+    # every Schur sum is used by exactly one core slot
+    sch = a["core_sch"]
+    used = np.sort(sch[sch >= 0])
+    assert np.array_equal(used, np.arange(plan.nsch))
 
 
 def test_mechanism_tables_match_the_oracle():
