@@ -29,4 +29,7 @@ def __getattr__(name):
     if name in ("BatchedAlgorithm",):
         from . import batched
         return getattr(batched, name)
+    if name in ("BatchedAdjoint", "integrate"):
+        from . import adjoint
+        return getattr(adjoint, name)
     raise AttributeError(name)

@@ -82,7 +82,17 @@ class BatchDesc(ct.Structure):
                 ("k_stride", ct.c_int32), ("dt", ct.c_double), ("rtol", ct.c_double),
                 ("Y", ct.c_void_p), ("k", ct.c_void_p), ("kmul", ct.c_void_p), ("atol", ct.c_void_p),
                 ("mass", ct.c_void_p), ("g", ct.c_void_p), ("work", ct.c_void_p),
-                ("queue", ct.c_void_p)]
+                ("queue", ct.c_void_p), ("hlog", ct.c_void_p), ("nacc", ct.c_void_p),
+                ("hcap", ct.c_int32), ("pad_", ct.c_int32)]
+
+
+class AdjointDesc(ct.Structure):
+    _fields_ = [("C", ct.c_int32), ("group_size", ct.c_int32), ("ngroups", ct.c_int32),
+                ("k_stride", ct.c_int32), ("hcap", ct.c_int32), ("pad_", ct.c_int32),
+                ("ws_stride", ct.c_int64),
+                ("Y", ct.c_void_p), ("k", ct.c_void_p), ("kmul", ct.c_void_p), ("mass", ct.c_void_p),
+                ("hlog", ct.c_void_p), ("nacc", ct.c_void_p), ("lam", ct.c_void_p), ("kbar", ct.c_void_p),
+                ("ws", ct.c_void_p), ("queue", ct.c_void_p)]
 
 
 vp = ct.c_void_p
@@ -112,6 +122,8 @@ _SIGS = {
     "fode_batched_graph_create": (i32, [vp, vp, P(CtrlParams), P(BatchDesc), i32, P(vp)]),
     "fode_graph_launch": (i32, [vp, vp]),
     "fode_graph_destroy": (i32, [vp]),
+    "fode_adjoint_interval": (i32, [vp, vp, vp, P(CtrlParams), P(AdjointDesc), vp]),
+    "fode_adjoint_ws_blocks": (i32, [vp, vp, vp, i32, i32, P(i32)]),
 }
 
 for _name, (_res, _args) in _SIGS.items():

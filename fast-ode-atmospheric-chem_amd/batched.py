@@ -57,8 +57,8 @@ class BatchedAlgorithm:
     def new_memory(self, C):
         return GroupMemories(-(-C // self.group_size))
 
-    def runner(self, system, Y, dt, atol, rtol, memory=None, graph=True):
-        return BatchedRunner(self, system, Y, dt, atol, rtol, memory, graph)
+    def runner(self, system, Y, dt, atol, rtol, memory=None, graph=True, record=False):
+        return BatchedRunner(self, system, Y, dt, atol, rtol, memory, graph, record)
 
     def solve(self, system, Y, dt, atol, rtol, counts: Counts, memory=None, kmul=None):
         """One host interval for every group. Returns (Y_new, statuses), the
@@ -79,7 +79,8 @@ class BatchedRunner:
     """Device buffers and (optionally) the captured interval graph of one
     batch; ``step(kmul)`` advances every group by one interval."""
 
-    def __init__(self, balg: BatchedAlgorithm, system, Y, dt, atol, rtol, memory=None, graph=True):
+    def __init__(self, balg: BatchedAlgorithm, system, Y, dt, atol, rtol, memory=None, graph=True,
+                 record=False):
         if system.has_constraints and balg.tableau.name != "rodas3":
             raise ValueError("constrained systems need RODAS3")
         if system.has_constraints:
@@ -110,6 +111,12 @@ class BatchedRunner:
             dt=float(dt), rtol=float(rtol), Y=self.Y.data_ptr(), k=k.data_ptr(),
             kmul=self.kmul.data_ptr(), atol=self.atol.data_ptr(), mass=system.mass_dev.data_ptr(),
             g=self.memory.dev.data_ptr(), work=self.work.data_ptr(), queue=self.queue.data_ptr())
+        self.hcap = int(balg.alg.max_steps)
+        if record:
+            # accepted steps of the last call: H per group, and their number
+            self.hlog = torch.zeros((self.ngroups, self.hcap), dtype=F64, device=self.Y.device)
+            self.nacc = torch.zeros(self.ngroups, dtype=torch.int32, device=self.Y.device)
+            self._point_record(self.hlog, self.nacc)
         self.graph = None
         if graph:
             h = ct.c_void_p()
@@ -118,6 +125,24 @@ class BatchedRunner:
                                                     int(balg.blocks_per_cu), ct.byref(h)),
                     "fode_batched_graph_create")
             self.graph = h
+
+    def record_into(self, hlog, nacc):
+        """Record the accepted steps of the following calls into hlog
+        (ngroups, max_steps) and nacc (ngroups,) (eager launches only: a
+        captured graph keeps the arrays it was built with)."""
+        if self.graph is not None:
+            raise RuntimeError("re-pointing step records needs eager launches (graph=False)")
+        self._point_record(hlog, nacc)
+
+    def _point_record(self, hlog, nacc):
+        if tuple(hlog.shape) != (self.ngroups, self.hcap) or tuple(nacc.shape) != (self.ngroups,):
+            raise ValueError("record arrays have the wrong shape")
+        if hlog.dtype != F64 or nacc.dtype != torch.int32 or not (hlog.is_contiguous() and nacc.is_contiguous()):
+            raise ValueError("record arrays must be contiguous float64 / int32")
+        self.desc.hlog = hlog.data_ptr()
+        self.desc.nacc = nacc.data_ptr()
+        self.desc.hcap = self.hcap
+        self._rec = (hlog, nacc)
 
     def set_kmul(self, kmul):
         if kmul is None:

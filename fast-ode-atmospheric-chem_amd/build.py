@@ -8,7 +8,7 @@ import sys
 from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
-SRCS = [HERE / "csrc" / "fastode_hip.hip", HERE / "csrc" / "batched.hip"]
+SRCS = [HERE / "csrc" / "fastode_hip.hip", HERE / "csrc" / "batched.hip", HERE / "csrc" / "adjoint.hip"]
 LIB = HERE / "libfastode_hip.so"
 ARCH = os.environ.get("FODE_OFFLOAD_ARCH", "gfx950")
 

@@ -15,7 +15,7 @@
 
 #include <string>
 
-#include "engine.hpp"
+#include "cell.hpp"
 
 using namespace fode;
 
@@ -31,16 +31,6 @@ int bfail(const std::string& m) {
   return -1;
 }
 
-// LDS layout of the batched kernel, in doubles from the dynamic base.
-struct Layout {
-  int tables;      // uint32 words of the staged tables (rounded to doubles)
-  int engine;      // engine region offset
-  int cell;        // cell region offset
-  int rd;          // rates / rate partials: the inverse scratch when it fits
-  int jval;        // Jacobian values: the inverse-block region (dead until factored)
-  int total;
-};
-
 struct BatchArgs {
   PlanDev P;
   MechDev M;
@@ -48,39 +38,6 @@ struct BatchArgs {
   fode_batch_desc b;
   Layout lay;
 };
-
-__host__ __device__ inline int cell_region_doubles(int n, int R, int S) {
-  return 4 * n + S * n + R + 4 + 2;
-}
-
-// stage: copy the task tables into LDS (when they fit beside the cell's data).
-Layout make_layout(const PlanDev& P, const MechDev& M, int S, bool stage) {
-  Layout l;
-  const int words = table_words(P.n_f_tasks, P.f_nst, P.n_f_pairs, P.n_sf_tasks, P.sf_nst,
-                                P.n_sb_tasks, P.sb_nst, P.n_s_pairs);
-  l.tables = stage ? (words + 1) / 2 : 0;
-  l.engine = l.tables;
-  int end = l.engine + engine_lds_doubles(P.lds_len, P.mpad, P.tile);
-  // Rates (R) and rate partials (R*P) are needed between the solves, when
-  // only the inverse scratch (scr) is dead; they get a region of their own if
-  // scr is too small.
-  const int scr = P.tile * NT;
-  const int scr_at = l.engine + (P.lds_len + 4 * P.mpad + 2 + 2 * NT);
-  const int need_rd = M.R * M.P > M.R ? M.R * M.P : M.R;
-  if (need_rd <= scr) {
-    l.rd = scr_at;
-  } else {
-    l.rd = end;
-    end += need_rd;
-  }
-  // Jacobian values are dead once assembled, before the inverse blocks
-  // (dL, dU) are written.
-  l.jval = scr_at + scr;
-  if (l.jval + M.nnz > end) end = l.jval + M.nnz;
-  l.cell = end;
-  l.total = l.cell + cell_region_doubles(M.n, M.R, S);
-  return l;
-}
 
 // Tables in LDS when everything fits in one workgroup's LDS, else in memory.
 Layout choose_layout(const PlanDev& P, const MechDev& M, int S) {
@@ -92,112 +49,18 @@ Layout choose_layout(const PlanDev& P, const MechDev& M, int S) {
   return make_layout(P, M, S, false);
 }
 
-struct CellLds {
-  double* y;      // n
-  double* f0;     // n   F(Y)
-  double* fr;     // n   forcing of a later stage; E at the end
-  double* w;      // n   stage point; the new state at the end
-  double* K;      // S*n
-  double* kc;     // R   rate constants
-  double* rd;     // R*P rate partials / R rates
-  double* jv;     // nnz Jacobian values
-  double* red;    // 4
-  int* flags;     // 4 ints
-};
-
-struct JLds {
-  const double* J;
-  __device__ double operator()(int e) const { return J[e]; }
-};
-
-__device__ inline void rates_into(const MechDev& M, const double* y, const double* kc, double* rates) {
-  for (int j = threadIdx.x; j < M.R; j += NT) rates[j] = rate(M, y, kc, j);
-  __syncthreads();
-}
-
-__device__ inline void forcing_into(const MechDev& M, const double* y, const double* rates, double* out) {
-  for (int i = threadIdx.x; i < M.n; i += NT) out[i] = forcing(M, y, rates, i);
-  __syncthreads();
-}
-
 // One attempt on one cell. Returns the cell's error partial sum (thread 0).
 template <int TILE>
 __device__ double cell_attempt(const BatchArgs& A, const Tables& T, const EngineLds& L,
                                const CellLds& V, const fode_group_ctrl& g, int cell, double* ynew_out) {
-  const PlanDev& P = A.P;
   const MechDev& M = A.M;
-  const fode_ctrl_params& p = A.p;
   const int n = M.n, tid = threadIdx.x;
   const double* yg = A.b.Y + (size_t)cell * n;
   const double* kg = A.b.k + (size_t)cell * A.b.k_stride;
   for (int i = tid; i < n; i += NT) V.y[i] = yg[i];
   for (int j = tid; j < M.R; j += NT) V.kc[j] = kg[j] * A.b.kmul[j];
   __syncthreads();
-  // F(Y) (fastode/solvers.py:345) and the Jacobian (:347), recomputed per
-  // attempt: they are the same values a retry reuses.
-  rates_into(M, V.y, V.kc, V.rd);
-  forcing_into(M, V.y, V.rd, V.f0);
-  for (int jp = tid; jp < M.R * M.P; jp += NT) V.rd[jp] = drate(M, V.y, V.kc, jp / M.P, jp % M.P);
-  __syncthreads();
-  const JFromD jd{M, V.rd};
-  for (int e = tid; e < M.nnz; e += NT) V.jv[e] = jd(e);
-  __syncthreads();
-  double a[TILE][TILE];
-  assemble<TILE>(P, JLds{V.jv}, g.alpha, A.b.mass, false, L.lds, a);
-  factor_cell<TILE>(P, T, L, a);
-  // Stages (fastode/solvers.py:54-86).
-  double* vec = L.lds + P.vec;
-  const int S = p.stages;
-  const bool dae = p.constrained != 0;
-  const double* forcing_src = V.f0;
-  for (int r = 0; r < S; ++r) {
-    if (r > 0 && p.new_forcing[r]) {
-      for (int i = tid; i < n; i += NT) {
-        double v = V.y[i];
-        for (int s = 0; s < r; ++s) {
-          const double as = p.a[r * 4 + s];
-          if (as != 0.0) v = v + as * V.K[s * n + i];
-        }
-        V.w[i] = v;
-      }
-      __syncthreads();
-      rates_into(M, V.w, V.kc, V.rd);
-      forcing_into(M, V.w, V.rd, V.fr);
-      forcing_src = V.fr;
-    }
-    // right-hand side of stage r, gathered into the solve vector (new order)
-    for (int q = tid; q < P.c + P.mpad; q += NT) {
-      double v = 0.0;
-      if (q < n) {
-        const int i = P.perm[q];
-        v = forcing_src[i];
-        for (int s = 0; s < r; ++s) {
-          const double coef = p.c[r * 4 + s] / g.H;
-          if (coef != 0.0) {
-            const double ks = dae ? V.K[s * n + i] * A.b.mass[i] : V.K[s * n + i];
-            v = v + coef * ks;
-          }
-        }
-      }
-      vec[q] = v;
-    }
-    solve_cell<TILE>(P, T, L, a);
-    for (int q = tid; q < n; q += NT) V.K[r * n + P.perm[q]] = vec[q];
-    __syncthreads();
-  }
-  // Ynew and the embedded error; E kept in fr.
-  for (int i = tid; i < n; i += NT) {
-    double y = V.y[i], e = 0.0;
-    for (int r = 0; r < S; ++r) {
-      const double kr = V.K[r * n + i];
-      if (p.m[r] != 0.0) y = y + p.m[r] * kr;
-      if (p.e[r] != 0.0) e = e + p.e[r] * kr;
-    }
-    V.w[i] = y;
-    V.fr[i] = e;
-    ynew_out[i] = y;
-  }
-  __syncthreads();
+  cell_stages<TILE>(A.P, T, M, A.p, A.b.mass, L, V, g.H, g.alpha, ynew_out);
   return cell_err_sum(V.y, V.w, V.fr, A.b.atol, A.b.rtol, n, V.red);
 }
 
@@ -207,17 +70,7 @@ __global__ __launch_bounds__(NT) void k_batched(BatchArgs A) {
   const int n = A.M.n, S = A.p.stages;
   const Tables T = A.lay.tables > 0 ? stage_tables(A.P, reinterpret_cast<uint32_t*>(smem)) : A.P.t;
   const EngineLds L = carve(smem + A.lay.engine, A.P);
-  CellLds V;
-  V.rd = smem + A.lay.rd;
-  V.jv = smem + A.lay.jval;
-  V.y = smem + A.lay.cell;
-  V.f0 = V.y + n;
-  V.fr = V.f0 + n;
-  V.w = V.fr + n;
-  V.K = V.w + n;
-  V.kc = V.K + S * n;
-  V.red = V.kc + A.M.R;
-  V.flags = reinterpret_cast<int*>(V.red + 4);
+  const CellLds V = carve_cell(smem, A.lay, n, A.M.R, S);
   __shared__ fode_group_ctrl gs;
   const int tid = threadIdx.x;
   const fode_batch_desc& B = A.b;
@@ -236,6 +89,7 @@ __global__ __launch_bounds__(NT) void k_batched(BatchArgs A) {
       ctrl_outer(A.p, gs);
     }
     __syncthreads();
+    int nrec = 0;   // accepted steps recorded (thread 0)
     while (gs.action != FODE_FINISH) {
       double sum = 0.0;
       for (int c = 0; c < cells; ++c) {
@@ -243,7 +97,11 @@ __global__ __launch_bounds__(NT) void k_batched(BatchArgs A) {
         const double cs = cell_attempt<TILE>(A, T, L, V, gs, c0 + c, ynew);
         sum = sum + cs;
       }
-      if (tid == 0) ctrl_after(A.p, gs, err_from_sum(sum, (double)cells * n));
+      if (tid == 0) {
+        const double h_used = gs.H;
+        ctrl_after(A.p, gs, err_from_sum(sum, (double)cells * n));
+        if (gs.accepted && B.hlog && nrec < B.hcap) B.hlog[(size_t)group * B.hcap + nrec++] = h_used;
+      }
       __syncthreads();
       if (gs.accepted) {
         const long N = (long)cells * n;
@@ -268,6 +126,7 @@ __global__ __launch_bounds__(NT) void k_batched(BatchArgs A) {
         gs.fails += 1;
       }
       B.g[group] = gs;
+      if (B.nacc) B.nacc[group] = nrec;
     }
     __syncthreads();
   }
@@ -377,6 +236,8 @@ int fode_batched_interval(void* plan, void* mech, const fode_ctrl_params* p, con
   if (p->stages < 1 || p->stages > 4) return bfail("fode_batched_interval: bad tableau");
   if (!b->Y || !b->k || !b->kmul || !b->atol || !b->mass || !b->g || !b->work || !b->queue)
     return bfail("fode_batched_interval: null array");
+  if ((b->hlog != nullptr) != (b->nacc != nullptr) || (b->hlog && b->hcap < p->max_steps))
+    return bfail("fode_batched_interval: hlog and nacc go together, with hcap >= max_steps");
   BatchArgs A;
   A.P = *static_cast<const PlanDev*>(fode_internal_plan(plan));
   A.M = *static_cast<const MechDev*>(fode_internal_mech(mech));

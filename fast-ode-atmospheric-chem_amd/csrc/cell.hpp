@@ -1,0 +1,178 @@
+// One Rosenbrock attempt on one cell, fused in one workgroup: the cell's
+// rates, forcing, Jacobian, stage matrix and its LU, the stage solves and the
+// stage combination (fastode/solvers.py:54-86 with the System calls of
+// fastode/chem.py:81-134), all in LDS and registers.  Shared by the batched
+// persistent kernel and the adjoint's recomputation, so that both produce
+// the same bits.
+#pragma once
+#include "engine.hpp"
+
+namespace fode {
+
+// LDS layout of a workgroup, in doubles from the dynamic base.
+struct Layout {
+  int tables;      // uint32 words of the staged tables (rounded to doubles)
+  int engine;      // engine region offset
+  int cell;        // cell region offset
+  int rd;          // rates / rate partials: the inverse scratch when it fits
+  int jval;        // Jacobian values: the inverse-block region (dead until factored)
+  int total;
+};
+
+__host__ __device__ inline int cell_region_doubles(int n, int R, int S) {
+  return 4 * n + S * n + R + 4 + 2;
+}
+
+// stage: copy the task tables into LDS (when they fit beside the cell's data).
+inline Layout make_layout(const PlanDev& P, const MechDev& M, int S, bool stage) {
+  Layout l;
+  const int words = table_words(P.n_f_tasks, P.f_nst, P.n_f_pairs, P.n_sf_tasks, P.sf_nst,
+                                P.n_sb_tasks, P.sb_nst, P.n_s_pairs);
+  l.tables = stage ? (words + 1) / 2 : 0;
+  l.engine = l.tables;
+  int end = l.engine + engine_lds_doubles(P.lds_len, P.mpad, P.tile);
+  // Rates (R) and rate partials (R*P) are needed between the solves, when
+  // only the inverse scratch (scr) is dead; they get a region of their own if
+  // scr is too small.
+  const int scr = P.tile * NT;
+  const int scr_at = l.engine + (P.lds_len + 4 * P.mpad + 2 + 2 * NT);
+  const int need_rd = M.R * M.P > M.R ? M.R * M.P : M.R;
+  if (need_rd <= scr) {
+    l.rd = scr_at;
+  } else {
+    l.rd = end;
+    end += need_rd;
+  }
+  // Jacobian values are dead once assembled, before the inverse blocks
+  // (dL, dU) are written.
+  l.jval = scr_at + scr;
+  if (l.jval + M.nnz > end) end = l.jval + M.nnz;
+  l.cell = end;
+  l.total = l.cell + cell_region_doubles(M.n, M.R, S);
+  return l;
+}
+
+struct CellLds {
+  double* y;      // n
+  double* f0;     // n   F(Y)
+  double* fr;     // n   forcing of a later stage; E at the end
+  double* w;      // n   stage point; the new state at the end
+  double* K;      // S*n
+  double* kc;     // R   rate constants
+  double* rd;     // R*P rate partials / R rates
+  double* jv;     // nnz Jacobian values
+  double* red;    // 4
+  int* flags;     // 4 ints
+};
+
+__device__ inline CellLds carve_cell(double* smem, const Layout& lay, int n, int R, int S) {
+  CellLds V;
+  V.rd = smem + lay.rd;
+  V.jv = smem + lay.jval;
+  V.y = smem + lay.cell;
+  V.f0 = V.y + n;
+  V.fr = V.f0 + n;
+  V.w = V.fr + n;
+  V.K = V.w + n;
+  V.kc = V.K + S * n;
+  V.red = V.kc + R;
+  V.flags = reinterpret_cast<int*>(V.red + 4);
+  return V;
+}
+
+struct JLds {
+  const double* J;
+  __device__ double operator()(int e) const { return J[e]; }
+};
+
+__device__ inline void rates_into(const MechDev& M, const double* y, const double* kc, double* rates) {
+  for (int j = threadIdx.x; j < M.R; j += NT) rates[j] = rate(M, y, kc, j);
+  __syncthreads();
+}
+
+__device__ inline void forcing_into(const MechDev& M, const double* y, const double* rates, double* out) {
+  for (int i = threadIdx.x; i < M.n; i += NT) out[i] = forcing(M, y, rates, i);
+  __syncthreads();
+}
+
+// Rate partials at V.y into V.rd, then the Jacobian values into V.jv.
+__device__ inline void jacobian_into(const MechDev& M, const CellLds& V) {
+  for (int jp = threadIdx.x; jp < M.R * M.P; jp += NT) V.rd[jp] = drate(M, V.y, V.kc, jp / M.P, jp % M.P);
+  __syncthreads();
+  const JFromD jd{M, V.rd};
+  for (int e = threadIdx.x; e < M.nnz; e += NT) V.jv[e] = jd(e);
+  __syncthreads();
+}
+
+// One attempt of step H (alpha = 1/(H gamma)) from the state in V.y with the
+// rate constants in V.kc: stages in V.K, the new state in V.w (and
+// ynew_out when given), the embedded error E in V.fr.
+template <int TILE>
+__device__ void cell_stages(const PlanDev& P, const Tables& T, const MechDev& M, const fode_ctrl_params& p,
+                            const double* mass, const EngineLds& L, const CellLds& V, double H,
+                            double alpha, double* ynew_out) {
+  const int n = M.n, tid = threadIdx.x;
+  // F(Y) (fastode/solvers.py:345) and the Jacobian (:347), recomputed per
+  // attempt: they are the same values a retry reuses.
+  rates_into(M, V.y, V.kc, V.rd);
+  forcing_into(M, V.y, V.rd, V.f0);
+  jacobian_into(M, V);
+  double a[TILE][TILE];
+  assemble<TILE>(P, JLds{V.jv}, alpha, mass, false, L.lds, a);
+  factor_cell<TILE>(P, T, L, a);
+  // Stages (fastode/solvers.py:54-86).
+  double* vec = L.lds + P.vec;
+  const int S = p.stages;
+  const bool dae = p.constrained != 0;
+  const double* forcing_src = V.f0;
+  for (int r = 0; r < S; ++r) {
+    if (r > 0 && p.new_forcing[r]) {
+      for (int i = tid; i < n; i += NT) {
+        double v = V.y[i];
+        for (int s = 0; s < r; ++s) {
+          const double as = p.a[r * 4 + s];
+          if (as != 0.0) v = v + as * V.K[s * n + i];
+        }
+        V.w[i] = v;
+      }
+      __syncthreads();
+      rates_into(M, V.w, V.kc, V.rd);
+      forcing_into(M, V.w, V.rd, V.fr);
+      forcing_src = V.fr;
+    }
+    // right-hand side of stage r, gathered into the solve vector (new order)
+    for (int q = tid; q < P.c + P.mpad; q += NT) {
+      double v = 0.0;
+      if (q < n) {
+        const int i = P.perm[q];
+        v = forcing_src[i];
+        for (int s = 0; s < r; ++s) {
+          const double coef = p.c[r * 4 + s] / H;
+          if (coef != 0.0) {
+            const double ks = dae ? V.K[s * n + i] * mass[i] : V.K[s * n + i];
+            v = v + coef * ks;
+          }
+        }
+      }
+      vec[q] = v;
+    }
+    solve_cell<TILE>(P, T, L, a);
+    for (int q = tid; q < n; q += NT) V.K[r * n + P.perm[q]] = vec[q];
+    __syncthreads();
+  }
+  // Ynew and the embedded error; E kept in fr.
+  for (int i = tid; i < n; i += NT) {
+    double y = V.y[i], e = 0.0;
+    for (int r = 0; r < S; ++r) {
+      const double kr = V.K[r * n + i];
+      if (p.m[r] != 0.0) y = y + p.m[r] * kr;
+      if (p.e[r] != 0.0) e = e + p.e[r] * kr;
+    }
+    V.w[i] = y;
+    V.fr[i] = e;
+    if (ynew_out) ynew_out[i] = y;
+  }
+  __syncthreads();
+}
+
+}  // namespace fode

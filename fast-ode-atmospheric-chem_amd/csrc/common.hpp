@@ -43,6 +43,9 @@ struct MechDev {
   const int* jac_ptr; const int* jac_src; const double* jac_coef; const double* jac_const;
   const int* con_rows; const double* con_coef; const double* con_total;
   const int* con_of_row;   // n: constraint index of a row or -1
+  // transposes built by fode_mech_create (for the adjoint):
+  const int* st_ptr; const int* st_sp; const double* st_coef;   // S^T by reaction (R+1)
+  const int* sp_ptr; const int* sp_jp;   // reactant slots j*P+p of each species (n+1)
 };
 
 }  // namespace fode

@@ -333,6 +333,38 @@ int fode_mech_create(const fode_mech_desc* d, void** out) {
     con_of[d->con_rows[c]] = c;
   }
   rc |= upload(con_of.data(), con_of.size(), (int**)&m.con_of_row, h->owned);
+  {
+    // S^T by reaction from the species-major forcing tables, and the reactant
+    // slots of each species, both in ascending order.
+    const int nz = d->rhs_ptr[d->n];
+    std::vector<int> st_ptr(d->R + 1, 0), st_sp(nz);
+    std::vector<double> st_coef(nz);
+    for (int q = 0; q < nz; ++q) {
+      if (d->rhs_rxn[q] < 0 || d->rhs_rxn[q] >= d->R) { delete h; return fail("fode_mech_create: bad rhs_rxn"); }
+      ++st_ptr[d->rhs_rxn[q] + 1];
+    }
+    for (int j = 0; j < d->R; ++j) st_ptr[j + 1] += st_ptr[j];
+    std::vector<int> fill(st_ptr.begin(), st_ptr.end() - 1);
+    for (int i = 0; i < d->n; ++i)
+      for (int q = d->rhs_ptr[i]; q < d->rhs_ptr[i + 1]; ++q) {
+        const int at = fill[d->rhs_rxn[q]]++;
+        st_sp[at] = i;
+        st_coef[at] = d->rhs_coef[q];
+      }
+    std::vector<int> sp_ptr(d->n + 1, 0), sp_jp;
+    for (int jp = 0; jp < d->R * d->P; ++jp)
+      if (d->react[jp] >= 0 && d->react[jp] < d->n) ++sp_ptr[d->react[jp] + 1];
+    for (int i = 0; i < d->n; ++i) sp_ptr[i + 1] += sp_ptr[i];
+    sp_jp.resize(sp_ptr[d->n]);
+    std::vector<int> sfill(sp_ptr.begin(), sp_ptr.end() - 1);
+    for (int jp = 0; jp < d->R * d->P; ++jp)
+      if (d->react[jp] >= 0 && d->react[jp] < d->n) sp_jp[sfill[d->react[jp]]++] = jp;
+    rc |= upload(st_ptr.data(), st_ptr.size(), (int**)&m.st_ptr, h->owned);
+    rc |= upload(st_sp.data(), st_sp.size(), (int**)&m.st_sp, h->owned);
+    rc |= upload(st_coef.data(), st_coef.size(), (double**)&m.st_coef, h->owned);
+    rc |= upload(sp_ptr.data(), sp_ptr.size(), (int**)&m.sp_ptr, h->owned);
+    rc |= upload(sp_jp.data(), sp_jp.size(), (int**)&m.sp_jp, h->owned);
+  }
   if (d->ncon > 0) {
     rc |= upload(d->con_rows, (size_t)d->ncon, (int**)&m.con_rows, h->owned);
     rc |= upload(d->con_coef, (size_t)d->ncon * d->n, (double**)&m.con_coef, h->owned);

@@ -107,6 +107,22 @@ def mechanism_plan(mech) -> DevicePlan:
     return dp
 
 
+def transposed_plan(mech) -> DevicePlan:
+    """Plan of the transposed pattern (rows and columns swapped) under the
+    forward plan's ordering: its entries are the same Jacobian values, its
+    factorization is that of (alpha*M - J)^T, and its split matches the
+    forward plan's (the no-pivot fill of A^T is the transpose of A's)."""
+    dp = getattr(mech, "_plan_t", None)
+    if dp is None:
+        fwd = mechanism_plan(mech).plan
+        dp = DevicePlan(mech.n, mech.cols, mech.rows, order=fwd.perm, tile=fwd.tile)
+        t = dp.plan
+        if (t.c, t.m, t.tile, t.ns, t.nsch) != (fwd.c, fwd.m, fwd.tile, fwd.ns, fwd.nsch):
+            raise RuntimeError("the transposed plan's split differs from the forward plan's")
+        mech._plan_t = dp
+    return dp
+
+
 _DENSE_PLANS: dict = {}
 
 

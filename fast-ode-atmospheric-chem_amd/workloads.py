@@ -49,6 +49,7 @@ class Workload:
     diurnal: bool = False
     mass: np.ndarray = None
     Y0: np.ndarray = None       # (C, n)
+    weights: np.ndarray = None  # (n,) loss weights of the adjoint workload
 
     @property
     def n(self):
@@ -57,6 +58,17 @@ class Workload:
     @property
     def R(self):
         return self.mech.R
+
+    def kmul(self, interval):
+        """Per-reaction multiplier (R,) of an interval: the diurnal factor at
+        its start on photolysis reactions, 1 elsewhere."""
+        m = np.ones(self.R)
+        if self.diurnal:
+            m[self.photolysis] = diurnal_factor(interval * self.dt)
+        return m
+
+    def kmuls(self):
+        return np.stack([self.kmul(i) for i in range(self.n_intervals)])
 
     def k(self, interval, cells=slice(None)):
         """Rate constants (C_sel, R) for the given interval."""
@@ -129,3 +141,12 @@ def constrained100(C=65536, seed=4, n_intervals=24):
     rtol = 1.0e-3
     return Workload("mech100-constrained", mech, C, n_intervals, 300.0, rtol, np.full(100, rtol * FLOOR),
                     "rodas3", seed, k_base=syn.k, photolysis=syn.photolysis, mass=mass, Y0=Y0)
+
+
+def adjoint200(C=16384, seed=2, n_intervals=24):
+    """The diurnal 200-species workload on 16,384 cells over 24 intervals, with
+    the loss sum_c sum_s w_s log(Y_final[c, s] + 1e-17), w_s ~ U(0, 1)."""
+    w = diurnal200(C=C, seed=seed, n_intervals=n_intervals)
+    w.name = "mech200-adjoint"
+    w.weights = np.random.default_rng(seed + 5000).uniform(0.0, 1.0, size=w.n)
+    return w

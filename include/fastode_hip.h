@@ -177,6 +177,13 @@ typedef struct fode_batch_desc {
   fode_group_ctrl* g;      /* (ngroups) */
   double* work;            /* (C,n) */
   int32_t* queue;          /* 1 int32 */
+  /* Optional record of the accepted steps (for the adjoint), or NULL:
+   * hlog[group*hcap + i] = H of the group's i-th accepted step of this call,
+   * nacc[group] = their number (hcap >= the attempt budget, so it never
+   * overflows). */
+  double* hlog;
+  int32_t* nacc;
+  int32_t hcap, pad_;
 } fode_batch_desc;
 
 /* Launch one interval. blocks_per_cu <= 0 picks the occupancy limit. */
@@ -193,6 +200,41 @@ int fode_batched_graph_create(void* plan, void* mech, const fode_ctrl_params* p,
                               const fode_batch_desc* b, int blocks_per_cu, void** graph);
 int fode_graph_launch(void* graph, void* stream);
 int fode_graph_destroy(void* graph);
+
+/* ---- Discrete adjoint of one interval of the batched form.  No counterpart
+ * in the reference (its solver has no derivative); it differentiates the
+ * map that fode_batched_interval applies, with the accepted steps that call
+ * recorded (fode_batch_desc.hlog/nacc) held fixed: quantized steps are
+ * locally constant in the inputs.  Per cell (independent given the steps):
+ * the interval's accepted steps are recomputed from the checkpoint Y (the
+ * state at the interval's start) and their start states and stages kept in
+ * ws; then, last step first, (alpha*M - J)^T is factored with plan_t (the
+ * plan of the transposed pattern under the same ordering) and the stage
+ * adjoints, Hessian terms and rate-constant terms are accumulated:
+ *   lam  (C,n): in, dL/d(state after the interval's clamp); out, dL/dY;
+ *   kbar (C,R): += dL/d(log k) of this interval.
+ * ws holds blocks * ws_stride doubles, ws_stride >= max(nacc)*(stages+1)*n;
+ * fode_adjoint_ws_blocks tells the number of blocks.  Unconstrained systems
+ * only. */
+typedef struct fode_adjoint_desc {
+  int32_t C, group_size, ngroups, k_stride, hcap, pad_;
+  int64_t ws_stride;
+  const double* Y;         /* (C,n) checkpoint */
+  const double* k;         /* (C or 1, R) */
+  const double* kmul;      /* (R) */
+  const double* mass;      /* (n) */
+  const double* hlog;      /* (ngroups, hcap) */
+  const int32_t* nacc;     /* (ngroups) */
+  double* lam;             /* (C,n) */
+  double* kbar;            /* (C,R) */
+  double* ws;              /* (blocks, ws_stride) */
+  int32_t* queue;          /* 1 int32 */
+} fode_adjoint_desc;
+
+int fode_adjoint_interval(void* plan, void* plan_t, void* mech, const fode_ctrl_params* p,
+                          const fode_adjoint_desc* a, void* stream);
+/* Workgroups fode_adjoint_interval launches (size of ws in ws_stride units). */
+int fode_adjoint_ws_blocks(void* plan, void* plan_t, void* mech, int stages, int C, int* blocks);
 
 #ifdef __cplusplus
 }

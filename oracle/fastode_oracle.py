@@ -593,8 +593,127 @@ def fixed_steps(system, tab, Y, t_end, n_steps, counts=None):
     return Y
 
 
+# ----------------------------------------------------- tangent-linear model
+# Forward-mode derivative of the accepted steps of Algorithm.solve with the
+# step sequence held fixed (quantized steps are locally constant), for a
+# direction (dY, dlogk) in initial state and log rate constants.  It
+# differentiates stages (fastode/solvers.py:54-86) with the mass-action
+# rates and Jacobian of fastode/chem.py:81-102 and the clamp of
+# solvers.py:89-97; unconstrained mechanisms only.
+
+def _slots(mech, Y, pad):
+    """Y (C, n) gathered at the reactant slots (C, R, P), pad for no species."""
+    return np.concatenate([Y, np.full((Y.shape[0], 1), pad)], axis=1)[:, mech.react]
+
+
+def _partials(mech, k, X):
+    """d rate_j / d y_{react[j,p]} (C, R, P) from the slot values X."""
+    d = np.empty_like(X)
+    for p in range(mech.P):
+        prod = np.broadcast_to(k, X.shape[:2]).copy()
+        for q in range(mech.P):
+            if q != p:
+                prod *= X[:, :, q]
+        d[:, :, p] = prod
+    return d
+
+
+def forcing_tangent(mech, k, Y, dY, dlogk):
+    """d F(Y) along (dY, dlogk): J(Y) dY + S (r(Y) * dlogk)."""
+    X = _slots(mech, Y, 1.0)
+    dX = _slots(mech, dY, 0.0)
+    r = k * X.prod(axis=2)
+    v = (_partials(mech, k, X) * dX).sum(axis=2) + r * dlogk
+    return v @ mech.ST
+
+
+def jacobian_tangent(mech, k, Y, dY, dlogk, V):
+    """(d J(Y) along (dY, dlogk)) V for mass action: the Hessian of the
+    rates contracted with dY and V, plus the k-linear part of J."""
+    X = _slots(mech, Y, 1.0)
+    dX = _slots(mech, dY, 0.0)
+    VX = _slots(mech, V, 0.0)
+    P = mech.P
+    d = _partials(mech, k, X)
+    v = (d * VX).sum(axis=2) * dlogk
+    for p in range(P):
+        for q in range(P):
+            if p == q:
+                continue
+            prod = np.broadcast_to(k, X.shape[:2]).copy()
+            for s in range(P):
+                if s != p and s != q:
+                    prod *= X[:, :, s]
+            v = v + prod * VX[:, :, p] * dX[:, :, q]
+    return v @ mech.ST
+
+
+def tangent_step(mech, k, tab, Y, dY, dlogk, H):
+    """One step of size H from Y (the arithmetic of ``stages``) and its
+    tangent: returns (Ynew, dYnew)."""
+    system = ChemSystem(mech, k)
+    F0 = system.rhs(Y)
+    J = system.jac(Y)
+    LU = system.factor(1.0 / (H * tab.gamma), J)
+    K, dK = [], []
+    forcing, dforcing = F0, forcing_tangent(mech, k, Y, dY, dlogk)
+    for r in range(tab.stages):
+        if r > 0 and tab.new_forcing[r]:
+            Yr, dYr = Y.copy(), dY.copy()
+            for s in range(r):
+                if tab.a[r, s] != 0.0:
+                    Yr += tab.a[r, s] * K[s]
+                    dYr += tab.a[r, s] * dK[s]
+            forcing, dforcing = system.rhs(Yr), forcing_tangent(mech, k, Yr, dYr, dlogk)
+        rhs, drhs = forcing.copy(), dforcing.copy()
+        for s in range(r):
+            coef = tab.c[r, s] / H
+            if coef != 0.0:
+                rhs += coef * K[s]
+                drhs += coef * dK[s]
+        Kr = LU.solve(rhs)
+        drhs = drhs + jacobian_tangent(mech, k, Y, dY, dlogk, Kr)
+        K.append(Kr)
+        dK.append(LU.solve(drhs))
+    Ynew, dYnew = Y.copy(), dY.copy()
+    for r in range(tab.stages):
+        if tab.m[r] != 0.0:
+            Ynew += tab.m[r] * K[r]
+            dYnew += tab.m[r] * dK[r]
+    return Ynew, dYnew
+
+
+def tangent_linear(mech, k, kmuls, tab, Y0, dY0, dlogk, dt, atol, rtol, group_size=32, **options):
+    """Groups of ``group_size`` cells, each with its own carried Memory, over
+    len(kmuls) intervals (rate constants k * kmuls[i] in interval i), the
+    loop of fastode/ts1.py:196-216.  Returns (Y_final, dY_final, steps), steps
+    [interval][group] = the accepted step sizes."""
+    alg = Algorithm(tab, **options)
+    C = Y0.shape[0]
+    groups = [slice(g, min(g + group_size, C)) for g in range(0, C, group_size)]
+    mems = [alg.new_memory() for _ in groups]
+    Y, dY = Y0.copy(), dY0.copy()
+    steps = []
+    for m in kmuls:
+        kk = k * m[None, :]
+        row = []
+        for g, sl in enumerate(groups):
+            alg.trace = []
+            Yg, _ = alg.solve(ChemSystem(mech, kk[sl]), Y[sl], dt, atol, rtol, Counts(), mems[g])
+            hs = [h for (h, _, acc) in alg.trace if acc]
+            y, dy = Y[sl].copy(), dY[sl].copy()
+            for h in hs:
+                y, dy = tangent_step(mech, kk[sl], tab, y, dy, dlogk[sl], h)
+            dY[sl] = np.where(y > 0.0, dy, 0.0)
+            Y[sl] = Yg
+            row.append(hs)
+        steps.append(row)
+    alg.trace = None
+    return Y, dY, steps
+
+
 PINS = {
-    "Tableau/ROS3/RODAS3": "order conditions to 3, index-1 values, new_forcing flags (tests/test_oracle.py)",
+    "Tableau/ROS3/RODAS3":"order conditions to 3, index-1 values, new_forcing flags (tests/test_oracle.py)",
     "error_norm": "closed form on hand vectors; golden per-attempt err from the reference",
     "quantize": "divisor property h/2 < q <= h, n q = interval",
     "Algorithm.factor": "closed-form H211b/elementary values; golden step sequences",
@@ -604,4 +723,7 @@ PINS = {
     "project": "affine covariance; Robertson manifold; golden constrained-config Counts",
     "OracleMechanism": "hand RHS of Robertson; Jacobian vs central differences; golden Y",
     "SuperLU/LapackLU": "residual against numpy.linalg.solve",
+    "tangent_linear": "central differences of Algorithm.solve itself (fixed-step and controlled), "
+                      "closed-form dy/dy0 and dy/dlogk of y' = -k y; forcing/jacobian tangents vs "
+                      "complex-step derivatives of rhs/jac_values",
 }
