@@ -13,7 +13,10 @@ HBM before timing starts.
 The JSON line also carries the per-group plain form (one launch sequence
 per group and protocol call, the reference's call structure) measured on a
 sample of groups over the same intervals, the HBM roofline of the interval
-kernel and the CPU oracle's rate on the box's cores.
+kernel and the CPU oracle's rate on the box's cores.  The adjoint leg times
+the gradient of sum w_s log(Y_final + 1e-17) with respect to log k and Y0
+on 16,384 cells over 24 intervals against the plain batched forward of the
+same intervals ("adjoint_over_forward").
 """
 from __future__ import annotations
 
@@ -92,6 +95,52 @@ def cpu_baseline(C, n_groups, n_int):
                       f"intervals = {done} cell-intervals in {wall:.1f} s, spawn pool, 1 BLAS thread each"}
 
 
+def adjoint_leg(fa, torch, W, C, n_int):
+    """Gradient of sum_c sum_s w_s log(Y_final + 1e-17) with respect to the
+    log rate constants (C, R) and Y0, over n_int intervals: time of the plain
+    batched forward (graph replays) against the taped forward (checkpoints,
+    step records) plus the backward pass."""
+    from fastode_amd.adjoint import BatchedAdjoint
+    from fastode_amd.system import ChemSystem
+    dev = torch.device("cuda", 0)
+    wl = W.adjoint200(C=C, n_intervals=n_int)
+    k = torch.as_tensor(np.broadcast_to(wl.k_base, (C, wl.R)).copy(), device=dev)
+    kmuls = torch.as_tensor(wl.kmuls(), device=dev)
+    Y0 = torch.as_tensor(wl.Y0, device=dev)
+    w = torch.as_tensor(wl.weights, device=dev)
+    adj = BatchedAdjoint(wl.mech)
+    # warm: one interval of a small batch through both passes
+    Yw, tw = adj.forward(k[:64], Y0[:64], kmuls[:1], wl.dt, wl.atol, wl.rtol)
+    adj.backward(tw, torch.ones_like(Yw))
+    stream = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
+    run = fa.BatchedAlgorithm(fa.ROS3).runner(ChemSystem(wl.mech, k), Y0, wl.dt, wl.atol, wl.rtol, graph=True)
+    torch.cuda.synchronize()
+    ev[0].record(stream)
+    for i in range(n_int):
+        run.kmul.copy_(kmuls[i])
+        run.step_graph_only()
+    ev[1].record(stream)
+    Y, tape = adj.forward(k, Y0, kmuls, wl.dt, wl.atol, wl.rtol)
+    ev[2].record(stream)
+    Ybar = (w[None, :] / (Y + 1e-17)).expand_as(Y).contiguous()
+    ev[3].record(stream)
+    kbar, y0bar = adj.backward(tape, Ybar)
+    ev[4].record(stream)
+    torch.cuda.synchronize()
+    fwd = ev[0].elapsed_time(ev[1])
+    taped = ev[1].elapsed_time(ev[2])
+    bwd = ev[3].elapsed_time(ev[4])
+    same = bool(torch.equal(Y, run.Y))
+    nacc = torch.stack(tape.naccs).float()
+    return {"cells": C, "intervals": n_int, "forward_ms": round(fwd, 1), "taped_forward_ms": round(taped, 1),
+            "backward_ms": round(bwd, 1), "ratio_total": round((taped + bwd) / fwd, 3),
+            "ratio_backward": round(bwd / fwd, 3), "taped_state_bitwise_equal": same,
+            "gradients_finite": bool(torch.isfinite(kbar).all() and torch.isfinite(y0bar).all()),
+            "accepted_steps_per_group_interval": round(float(nacc.mean()), 3),
+            "note": "ratio_total = (taped forward + backward) / plain batched forward, same intervals"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,6 +153,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--blocks-per-cu", type=int, default=0)
+    ap.add_argument("--adjoint-cells", type=int, default=16384, help="0 skips the adjoint leg")
+    ap.add_argument("--adjoint-intervals", type=int, default=24)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: the groups shard "
@@ -220,6 +271,10 @@ def main():
                         "attempts_per_group_interval": round(pc.steps / (G * K), 3),
                         "states_bitwise_equal_to_batched": same}
         out["speedup_vs_plain"] = round(value / plain_value, 1)
+
+    if args.adjoint_cells > 0:
+        out["adjoint"] = adjoint_leg(fa, torch, W, args.adjoint_cells, args.adjoint_intervals)
+        out["adjoint_over_forward"] = out["adjoint"]["ratio_total"]
 
     if not args.no_cpu:
         cores = max(1, min(16, len(os.sched_getaffinity(0))))
