@@ -83,8 +83,6 @@ class BatchedRunner:
                  record=False):
         if system.has_constraints and balg.tableau.name != "rodas3":
             raise ValueError("constrained systems need RODAS3")
-        if system.has_constraints:
-            raise NotImplementedError("the batched form does not project constrained systems yet")
         self.balg = balg
         self.system = system
         self.Y = as_device(Y).clone()
@@ -101,7 +99,8 @@ class BatchedRunner:
         self.k = k
         self.kmul = torch.ones(system.mech.R, dtype=F64, device=self.Y.device)
         self.atol = as_device(atol)
-        self.work = torch.empty_like(self.Y)
+        # ynew per cell; the projection's corrections and saved state when constrained
+        self.work = torch.empty(((2 if system.has_constraints else 1) * C, n), dtype=F64, device=self.Y.device)
         self.queue = torch.zeros(1, dtype=torch.int32, device=self.Y.device)
         self.params = ctrl_params(balg.alg, balg.tableau, system.has_constraints)
         self.plan = mechanism_plan(system.mech)

@@ -108,7 +108,7 @@ __device__ inline void jacobian_into(const MechDev& M, const CellLds& V) {
 // rate constants in V.kc: stages in V.K, the new state in V.w (and
 // ynew_out when given), the embedded error E in V.fr.
 template <int TILE>
-__device__ void cell_stages(const PlanDev& P, const Tables& T, const MechDev& M, const fode_ctrl_params& p,
+__device__ __forceinline__ void cell_stages(const PlanDev& P, const Tables& T, const MechDev& M, const fode_ctrl_params& p,
                             const double* mass, const EngineLds& L, const CellLds& V, double H,
                             double alpha, double* ynew_out) {
   const int n = M.n, tid = threadIdx.x;

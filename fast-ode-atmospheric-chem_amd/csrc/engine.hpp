@@ -331,7 +331,7 @@ __device__ inline void core_solve(const double (&a)[TILE][TILE], const EngineLds
 
 // Full factorization of one cell from assembled values.
 template <int TILE>
-__device__ inline void factor_cell(const PlanDev& P, const Tables& T, const EngineLds& L,
+__device__ __forceinline__ void factor_cell(const PlanDev& P, const Tables& T, const EngineLds& L,
                                    double (&a)[TILE][TILE]) {
   __syncthreads();
   run_tasks(T.f_tasks, T.f_ptr, P.f_nst, T.f_pairs, L.lds);
@@ -343,7 +343,7 @@ __device__ inline void factor_cell(const PlanDev& P, const Tables& T, const Engi
 // Solve with the cell's factor; the right-hand side is already in the LDS
 // vector (new ordering, padding zero) and the solution is left there.
 template <int TILE>
-__device__ inline void solve_cell(const PlanDev& P, const Tables& T, const EngineLds& L,
+__device__ __forceinline__ void solve_cell(const PlanDev& P, const Tables& T, const EngineLds& L,
                                   const double (&a)[TILE][TILE]) {
   double* vec = L.lds + P.vec;
   __syncthreads();

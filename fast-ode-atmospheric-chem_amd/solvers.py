@@ -126,7 +126,7 @@ def project(system, Y, atol, rtol, counts: Counts, max_updates=10, tol=0.1,
     def residual(Z):
         G = system.rhs(Z)
         counts.rhs += 1
-        return G * alg          # differential rows zeroed
+        return torch.where(alg, G, torch.zeros_like(G))   # differential rows exactly 0
 
     def finite(A):
         return bool(torch.isfinite(A[:, alg]).all())

@@ -161,8 +161,9 @@ int fode_clamp(int C, int n, double* Y, const double* mass, int constrained, voi
  * semantics: each group (group_size consecutive cells; the last may be
  * shorter) keeps its own controller state and Memory in g[group] across
  * calls.  Rate constants of cell c are k[c*k_stride + j] * kmul[j]
- * (k_stride = 0 shares one row); work is a C*n scratch; queue is one int32
- * that the call resets itself.  Groups are taken from an atomic queue by
+ * (k_stride = 0 shares one row); work is a C*n scratch (2*C*n when
+ * p->constrained: the projection of fastode/solvers.py:100-178 runs per group
+ * at the start of each call); queue is one int32 that the call resets itself.  Groups are taken from an atomic queue by
  * persistent workgroups, so uneven groups balance.  The launch sequence is
  * stream-ordered and free of host synchronization and memsets, so it may be
  * captured in a hipGraph. */
@@ -175,7 +176,7 @@ typedef struct fode_batch_desc {
   const double* atol;      /* (n) */
   const double* mass;      /* (n) */
   fode_group_ctrl* g;      /* (ngroups) */
-  double* work;            /* (C,n) */
+  double* work;            /* (C,n); (2C,n) for a constrained tableau (projection) */
   int32_t* queue;          /* 1 int32 */
   /* Optional record of the accepted steps (for the adjoint), or NULL:
    * hlog[group*hcap + i] = H of the group's i-th accepted step of this call,

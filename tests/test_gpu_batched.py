@@ -82,6 +82,49 @@ def test_batched_matches_golden_mech200_uneven():
     assert worst <= 1.0
 
 
+@pytest.mark.parametrize("graph", [False, True])
+def test_batched_matches_golden_mech100_constrained(graph):
+    """RODAS3 with three conservation rows: the per-group projection runs in
+    the persistent kernel at the start of every call."""
+    worst, _ = replay_batched("mech100_constrained", graph)
+    print("batched mech100 constrained worst", worst)
+    assert worst <= 1.0
+
+
+def test_batched_constrained_equals_plain_bitwise():
+    """The constrained workload on a ragged batch (40 cells: 32 + 8), the
+    initial state pushed off its constraints so that the projection takes a
+    Newton update: batched and plain agree bit for bit, Counts included."""
+    import fastode_amd as fa
+    from fastode_amd import workloads as W
+    from fastode_amd.system import ChemSystem
+    wl = W.constrained100(C=40)
+    Y0 = wl.Y0.copy()
+    Y0[:, wl.mech.constraint_rows] *= 1.0 + 1e-3 * np.arange(1, 41)[:, None]
+    k = static_k(wl, np.arange(40))
+    system = ChemSystem(wl.mech, k, mass=wl.mass)
+    run = fa.BatchedAlgorithm(fa.RODAS3).runner(system, Y0, wl.dt, wl.atol, wl.rtol, graph=False)
+    alg = fa.Algorithm(fa.RODAS3)
+    mems = [alg.new_memory() for _ in range(2)]
+    cnt = [fa.Counts() for _ in range(2)]
+    Yp = Y0.copy()
+    for it in range(2):
+        run.step(None)
+        for gi, sl in enumerate([slice(0, 32), slice(32, 40)]):
+            Yp[sl], st = alg.solve(ChemSystem(wl.mech, k[sl], mass=wl.mass), Yp[sl], wl.dt, wl.atol,
+                                   wl.rtol, cnt[gi], mems[gi])
+            assert st == "converged"
+        assert np.array_equal(run.Y.cpu().numpy(), Yp)
+    rec = run.memory.read()
+    for gi in range(2):
+        assert list(rec["counts"][gi]) == [getattr(cnt[gi], f) for f in
+                                           ("steps", "accepted", "rejected", "rhs", "jac", "lu", "solves",
+                                            "newton", "failures", "unconverged", "proj_passes",
+                                            "proj_trials", "calls")]
+        assert rec["h_next"][gi] == mems[gi].h_next
+    assert rec["counts"][:, 10].min() >= 2          # the projection did a Newton pass
+
+
 def test_batched_equals_plain_bitwise_and_graph_replays_match_eager():
     """Groups of uneven size (the last one short), 3 intervals: the batched
     states equal the plain path's bit for bit, with eager work between graph
