@@ -32,4 +32,7 @@ def __getattr__(name):
     if name in ("BatchedAdjoint", "integrate"):
         from . import adjoint
         return getattr(adjoint, name)
+    if name in ("tune",):
+        from . import autotune
+        return getattr(autotune, name)
     raise AttributeError(name)

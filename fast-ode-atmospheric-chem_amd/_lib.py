@@ -68,9 +68,10 @@ class PlanDesc(ct.Structure):
                 ("ns", ct.c_int32), ("nsch", ct.c_int32), ("nnz", ct.c_int32), ("slot_stride", ct.c_int32),
                 ("n_f_tasks", ct.c_int32), ("f_nst", ct.c_int32), ("n_f_pairs", ct.c_int32),
                 ("n_sf_tasks", ct.c_int32), ("sf_nst", ct.c_int32), ("n_sb_tasks", ct.c_int32),
-                ("sb_nst", ct.c_int32), ("n_s_pairs", ct.c_int32),
+                ("sb_nst", ct.c_int32), ("n_s_pairs", ct.c_int32), ("n_sch_pairs", ct.c_int32),
                 ("sp_src", ct.c_void_p), ("sp_diag", ct.c_void_p), ("sp_row", ct.c_void_p),
-                ("core_src", ct.c_void_p), ("core_sch", ct.c_void_p), ("core_row", ct.c_void_p),
+                ("core_src", ct.c_void_p), ("sch_mask", ct.c_void_p), ("sch_base", ct.c_void_p),
+                ("sch_pairs", ct.c_void_p), ("core_row", ct.c_void_p),
                 ("f_tasks", ct.c_void_p), ("f_ptr", ct.c_void_p), ("f_pairs", ct.c_void_p),
                 ("sf_tasks", ct.c_void_p), ("sf_ptr", ct.c_void_p),
                 ("sb_tasks", ct.c_void_p), ("sb_ptr", ct.c_void_p),
@@ -83,7 +84,7 @@ class BatchDesc(ct.Structure):
                 ("Y", ct.c_void_p), ("k", ct.c_void_p), ("kmul", ct.c_void_p), ("atol", ct.c_void_p),
                 ("mass", ct.c_void_p), ("g", ct.c_void_p), ("work", ct.c_void_p),
                 ("queue", ct.c_void_p), ("hlog", ct.c_void_p), ("nacc", ct.c_void_p),
-                ("hcap", ct.c_int32), ("pad_", ct.c_int32)]
+                ("hcap", ct.c_int32), ("options", ct.c_int32)]
 
 
 class AdjointDesc(ct.Structure):

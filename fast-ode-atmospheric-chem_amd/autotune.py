@@ -1,0 +1,143 @@
+"""Autotuner of the batched form's launch configuration for a mechanism.
+
+The per-cell LU engine's cost depends on how the symbolic plan splits the
+ordered matrix into sparse pivots (level-scheduled LDS tasks) and the dense
+register-tiled core (the core's size picks the register tile TILE), and on
+whether the plan's tables are staged in LDS.  The tuner builds the distinct
+plans a range of core-density thresholds and core-size caps give, drops the
+ones whose workgroup does not fit in LDS, times one interval of the batched
+kernel on a sample of the caller's cells for each plan, with and without
+table staging, and keeps the fastest per cell-attempt.  Results are cached
+per (pattern, tableau, device) in a JSON file.
+
+``tune(...)`` installs the winner as the mechanism's plan, so every later
+solve (plain and batched) uses it.  Every candidate is a valid no-pivot plan
+of the same ordering; which one is chosen changes roundings only.
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+from dataclasses import asdict, dataclass
+from pathlib import Path
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from .device import DevicePlan
+from .system import F64, ChemSystem
+
+RHOS = (0.6, 0.7, 0.75, 0.8, 0.85, 0.9, 0.95)
+PLAN_FORMAT = 1          # bump when the plan tables or the engine change
+M_MAXES = (160, 144, 128)
+CACHE = Path(os.environ.get("FASTODE_AMD_TUNE_CACHE", Path.home() / ".cache" / "fastode_amd" / "tune.json"))
+
+
+@dataclass
+class Choice:
+    rho: float
+    m_max: int
+    stage_tables: bool
+    c: int = 0
+    tile: int = 0
+    us_per_cell_attempt: float = 0.0
+
+
+def pattern_key(mech, tableau, device_name):
+    h = hashlib.sha1()
+    h.update(np.asarray([mech.n, mech.nnz], dtype=np.int64).tobytes())
+    h.update(np.asarray(mech.rows, dtype=np.int64).tobytes())
+    h.update(np.asarray(mech.cols, dtype=np.int64).tobytes())
+    return f"{h.hexdigest()[:16]}:{tableau.name}:{device_name}:plan-format-{PLAN_FORMAT}"
+
+
+def candidates(mech):
+    """Distinct (rho, m_max) plans of the mechanism, by their split (c, tile)."""
+    out, seen = [], set()
+    for m_max in M_MAXES:
+        for rho in RHOS:
+            try:
+                dp = DevicePlan(mech.n, mech.rows, mech.cols, rho=rho, m_max=m_max)
+            except (ValueError, L.HipError):
+                continue
+            key = (dp.plan.c, dp.plan.tile)
+            if key not in seen:
+                seen.add(key)
+                out.append((rho, m_max, dp))
+    return out
+
+
+def _load_cache():
+    try:
+        return json.loads(CACHE.read_text())
+    except (OSError, ValueError):
+        return {}
+
+
+def _save_cache(d):
+    try:
+        CACHE.parent.mkdir(parents=True, exist_ok=True)
+        CACHE.write_text(json.dumps(d, indent=1, sort_keys=True))
+    except OSError:
+        pass
+
+
+def install(mech, choice: Choice):
+    """Make the choice the mechanism's plan (clears cached derived plans)."""
+    mech._plan = DevicePlan(mech.n, mech.rows, mech.cols, rho=choice.rho, m_max=choice.m_max)
+    if hasattr(mech, "_plan_t"):
+        del mech._plan_t
+    mech._tuned = choice
+    return mech._plan
+
+
+def tune(mech, k, Y, dt, atol, rtol, tableau=None, kmul=None, use_cache=True, verbose=False) -> Choice:
+    """Time the candidate plans on the sample (k (C, R) or (1, R), Y (C, n):
+    the caller's cells, C about one group per CU or more) over one interval
+    and install the fastest."""
+    from .batched import BatchedAlgorithm
+    from .tableaus import ROS3
+    tableau = tableau or ROS3
+    dev_name = torch.cuda.get_device_name(0)
+    key = pattern_key(mech, tableau, dev_name)
+    cache = _load_cache() if use_cache else {}
+    if key in cache:
+        choice = Choice(**cache[key])
+        install(mech, choice)
+        return choice
+    saved = getattr(mech, "_plan", None)
+    results = []
+    for rho, m_max, dp in candidates(mech):
+        mech._plan = dp
+        for stage in (True, False):
+            balg = BatchedAlgorithm(tableau, stage_tables=stage)
+            try:
+                system = ChemSystem(mech, k)
+                run = balg.runner(system, Y, dt, atol, rtol, graph=False)
+                ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                torch.cuda.synchronize()
+                ev0.record()
+                run.step(kmul)
+                ev1.record()
+                torch.cuda.synchronize()
+            except L.HipError:
+                continue
+            ms = ev0.elapsed_time(ev1)
+            att = float((run.memory.counts()[:, 0] * np.minimum(
+                balg.group_size, run.C - balg.group_size * np.arange(run.ngroups))).sum())
+            us = 1e3 * ms / max(att, 1.0)
+            results.append(Choice(rho, m_max, stage, dp.plan.c, dp.plan.tile, us))
+            if verbose:
+                print(f"tune: rho {rho} m_max {m_max} c {dp.plan.c} tile {dp.plan.tile} "
+                      f"stage {stage}: {us:.3f} us per cell-attempt")
+    if not results:
+        mech._plan = saved
+        raise RuntimeError("no candidate plan fits this device")
+    best = min(results, key=lambda r: r.us_per_cell_attempt)
+    install(mech, best)
+    if use_cache:
+        cache[key] = asdict(best)
+        _save_cache(cache)
+    return best

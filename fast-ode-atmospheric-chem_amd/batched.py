@@ -48,11 +48,15 @@ class BatchedAlgorithm:
 
     name = "batched-algorithm"
 
-    def __init__(self, tableau=ROS3, group_size=32, blocks_per_cu=0, **options):
+    def __init__(self, tableau=ROS3, group_size=32, blocks_per_cu=0, stage_tables=None, **options):
+        """stage_tables: copy the plan's tables into LDS when they fit (True),
+        never (False), or as the mechanism's tuned choice says (None; stage
+        when untuned)."""
         self.alg = Algorithm(tableau, **options)
         self.tableau = tableau
         self.group_size = group_size
         self.blocks_per_cu = blocks_per_cu
+        self.stage_tables = stage_tables
 
     def new_memory(self, C):
         return GroupMemories(-(-C // self.group_size))
@@ -109,7 +113,8 @@ class BatchedRunner:
             C=C, group_size=gs, ngroups=self.ngroups, k_stride=(0 if k.shape[0] == 1 else k.shape[1]),
             dt=float(dt), rtol=float(rtol), Y=self.Y.data_ptr(), k=k.data_ptr(),
             kmul=self.kmul.data_ptr(), atol=self.atol.data_ptr(), mass=system.mass_dev.data_ptr(),
-            g=self.memory.dev.data_ptr(), work=self.work.data_ptr(), queue=self.queue.data_ptr())
+            g=self.memory.dev.data_ptr(), work=self.work.data_ptr(), queue=self.queue.data_ptr(),
+            options=0 if self._stage(balg, system) else 1)
         self.hcap = int(balg.alg.max_steps)
         if record:
             # accepted steps of the last call: H per group, and their number
@@ -124,6 +129,13 @@ class BatchedRunner:
                                                     int(balg.blocks_per_cu), ct.byref(h)),
                     "fode_batched_graph_create")
             self.graph = h
+
+    @staticmethod
+    def _stage(balg, system):
+        if balg.stage_tables is not None:
+            return bool(balg.stage_tables)
+        tuned = getattr(system.mech, "_tuned", None)
+        return True if tuned is None else bool(tuned.stage_tables)
 
     def record_into(self, hlog, nacc):
         """Record the accepted steps of the following calls into hlog

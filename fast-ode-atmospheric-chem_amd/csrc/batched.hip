@@ -39,13 +39,15 @@ struct BatchArgs {
   Layout lay;
 };
 
-// Tables in LDS when everything fits in one workgroup's LDS, else in memory.
-Layout choose_layout(const PlanDev& P, const MechDev& M, int S) {
+// Tables in LDS when everything fits in one workgroup's LDS (and the caller
+// allows it), else in memory.
+Layout choose_layout(const PlanDev& P, const MechDev& M, int S, int options = 0) {
   int dev = 0, maxlds = 0;
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
   const Layout staged = make_layout(P, M, S, true);
-  if ((size_t)staged.total * sizeof(double) + sizeof(fode_group_ctrl) <= (size_t)maxlds) return staged;
+  if (!(options & 1) && (size_t)staged.total * sizeof(double) + sizeof(fode_group_ctrl) <= (size_t)maxlds)
+    return staged;
   return make_layout(P, M, S, false);
 }
 
@@ -453,7 +455,7 @@ int fode_batched_interval(void* plan, void* mech, const fode_ctrl_params* p, con
   if (A.P.n != A.M.n || A.P.nnz != A.M.nnz) return bfail("fode_batched_interval: plan and mechanism differ");
   A.p = *p;
   A.b = *b;
-  A.lay = choose_layout(A.P, A.M, p->stages);
+  A.lay = choose_layout(A.P, A.M, p->stages, b->options);
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   switch (A.P.tile) {
 #define X(T) case T: return launch_batched<T>(A, blocks_per_cu, s);

@@ -27,10 +27,11 @@ __host__ __device__ inline int cell_region_doubles(int n, int R, int S) {
 inline Layout make_layout(const PlanDev& P, const MechDev& M, int S, bool stage) {
   Layout l;
   const int words = table_words(P.n_f_tasks, P.f_nst, P.n_f_pairs, P.n_sf_tasks, P.sf_nst,
-                                P.n_sb_tasks, P.sb_nst, P.n_s_pairs);
+                                P.n_sb_tasks, P.sb_nst, P.n_s_pairs,
+                                P.n_sch_pairs);
   l.tables = stage ? (words + 1) / 2 : 0;
   l.engine = l.tables;
-  int end = l.engine + engine_lds_doubles(P.lds_len, P.mpad, P.tile);
+  int end = l.engine + engine_lds_doubles(P);
   // Rates (R) and rate partials (R*P) are needed between the solves, when
   // only the inverse scratch (scr) is dead; they get a region of their own if
   // scr is too small.

@@ -21,18 +21,21 @@ struct Tables {
   const uint2* sf_tasks; const int* sf_ptr;
   const uint2* sb_tasks; const int* sb_ptr;
   const uint32_t* s_pairs;
+  const uint32_t* sch_pairs;   // Schur pairs of the core entries, per thread
 };
 
 struct PlanDev {
   int n, c, m, tile, ns, nsch, nnz;
-  int lds_len, lu_len, mpad, slot_stride, vec;   // vec = ns + nsch
+  int lds_len, lu_len, mpad, slot_stride, vec;   // vec = ns
   int f_nst, sf_nst, sb_nst;
   // sizes (tasks, pairs) of the tables, for staging them into LDS
-  int n_f_tasks, n_f_pairs, n_sf_tasks, n_sb_tasks, n_s_pairs;
+  int n_f_tasks, n_f_pairs, n_sf_tasks, n_sb_tasks, n_s_pairs, n_sch_pairs;
   Tables t;                    // global-memory tables
   const int* sp_src; const int* sp_diag; const int* sp_row;
   const short* core_src;       // [NT][slot_stride] Jacobian entry, -1 fill, -2 identity
-  const short* core_sch;       // [NT][slot_stride] Schur sum index, -1 none
+  const uint32_t* sch_mask;    // [NT][4] bit s: slot s has a Schur sum
+  const int* sch_base;         // [NT + 1] Schur pair range of each thread
+  int sch_depth;               // most Schur sums of one thread
   const int* core_row;         // [mpad] old row of core row, -1 padding
   const int* perm;             // new -> old
 };

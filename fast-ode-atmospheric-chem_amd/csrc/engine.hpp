@@ -30,8 +30,14 @@ struct EngineLds {
   double* dU;     // [tile*NT]  inverse diagonal blocks of U, transposed
 };
 
-__host__ __device__ inline int engine_lds_doubles(int lds_len, int mpad, int tile) {
-  return lds_len + 4 * mpad + 2 + 2 * NT + 3 * tile * NT;
+// After lds: the pivot buffers (lbuf, ubuf, rbuf), then pb, scr, dL, dU, or
+// (while the Schur update runs, before any of these is live; the pivot
+// buffers are written as soon as a thread leaves it) the Schur sums,
+// thread-interleaved S[j * NT + t] from pb on.
+__host__ __device__ inline int engine_lds_doubles(const PlanDev& P) {
+  const int rest = 2 * NT + 3 * P.tile * NT;
+  const int sums = (P.sch_depth + 1) * NT;
+  return P.lds_len + 4 * P.mpad + 2 + (rest > sums ? rest : sums);
 }
 
 __device__ inline EngineLds carve(double* base, const PlanDev& P) {
@@ -48,11 +54,23 @@ __device__ inline EngineLds carve(double* base, const PlanDev& P) {
 }
 
 // ---------------------------------------------------------------- helpers
+// A double moved across lanes by a DPP pattern (both 32-bit halves).
+template <int CTRL>
+__device__ inline double dpp_mov(double x) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), CTRL, 0xF, 0xF, true);
+  return __hiloint2double(hi, lo);
+}
+
+// Sum over the 16 lanes of a DPP row (threads (tr, 0..15) of the engine
+// grid), the same bits in every lane: lane ^ 1 and lane ^ 2 by quad
+// permutes, then the other quad of the half row (half-row mirror), then the
+// other half row (row mirror).  Register-to-register, no LDS crossbar.
 __device__ inline double rowsum16(double x) {
-  x += __shfl_xor(x, 8, 16);
-  x += __shfl_xor(x, 4, 16);
-  x += __shfl_xor(x, 2, 16);
-  x += __shfl_xor(x, 1, 16);
+  x += dpp_mov<0xB1>(x);    // quad_perm [1, 0, 3, 2]
+  x += dpp_mov<0x4E>(x);    // quad_perm [2, 3, 0, 1]
+  x += dpp_mov<0x141>(x);   // row_half_mirror
+  x += dpp_mov<0x140>(x);   // row_mirror
   return x;
 }
 
@@ -104,17 +122,22 @@ __device__ inline Tables stage_tables(const PlanDev& P, uint32_t* w) {
   t.sb_tasks = reinterpret_cast<const uint2*>(take(P.t.sb_tasks, 2 * P.n_sb_tasks));
   t.sb_ptr = reinterpret_cast<const int*>(take(P.t.sb_ptr, P.sb_nst + 1));
   t.s_pairs = take(P.t.s_pairs, P.n_s_pairs);
+  p += (4 - ((p - w) & 3)) & 3;   // 16-byte groups of the Schur lists
+  t.sch_pairs = take(P.t.sch_pairs, P.n_sch_pairs);
   __syncthreads();
   return t;
 }
 
 __host__ __device__ inline int table_words(int n_f_tasks, int f_nst, int n_f_pairs, int n_sf_tasks,
-                                           int sf_nst, int n_sb_tasks, int sb_nst, int n_s_pairs) {
+                                           int sf_nst, int n_sb_tasks, int sb_nst, int n_s_pairs,
+                                           int n_sch_pairs) {
   int w = 2 * n_f_tasks + f_nst + 1 + n_f_pairs;
   w += w & 1;
   w += 2 * n_sf_tasks + sf_nst + 1;
   w += w & 1;
   w += 2 * n_sb_tasks + sb_nst + 1 + n_s_pairs;
+  w += (4 - (w & 3)) & 3;
+  w += n_sch_pairs;
   return w;
 }
 
@@ -135,6 +158,18 @@ __device__ inline double stage_entry(const JS& jv, int e, bool row_alg, bool dia
   return v;
 }
 
+// stage_entry without branches, for the unrolled core slots: every read is
+// made (entry 0 stands in for a missing one) and the result selected, so
+// that the slots' loads issue together.  Same bits as stage_entry.
+template <class JS>
+__device__ inline double stage_entry_select(const JS& jv, int e, bool row_alg, bool diag, double alpha,
+                                            double mass_d, bool alg_only) {
+  const double jr = jv(e >= 0 ? e : 0);
+  const double j = (alg_only && !row_alg) ? 0.0 : jr;
+  const double v = (e >= 0) ? -j : 0.0;
+  return diag ? v + alpha * mass_d : v;
+}
+
 template <int TILE, class JS>
 __device__ inline void assemble(const PlanDev& P, const JS& jv, double alpha, const double* mass,
                                 bool alg_only, double* lds, double (&a)[TILE][TILE]) {
@@ -144,7 +179,6 @@ __device__ inline void assemble(const PlanDev& P, const JS& jv, double alpha, co
     lds[s] = stage_entry(jv, P.sp_src[s], mass[P.sp_row[s]] == 0.0, d >= 0, alpha,
                          d >= 0 ? mass[d] : 0.0, alg_only);
   }
-  for (int s = tid; s < P.nsch; s += NT) lds[P.ns + s] = 0.0;   // Schur sums start at zero
   // the row of each of this thread's core rows: its mass (alg-only mask, diagonal)
   double rmass[TILE];
 #pragma unroll
@@ -159,22 +193,57 @@ __device__ inline void assemble(const PlanDev& P, const JS& jv, double alpha, co
     for (int tj = 0; tj < TILE; ++tj) {
       const int e = src[ti * TILE + tj];
       const bool diag = (ti == tj) && (tr == tc);
-      a[ti][tj] = (e == -2) ? 1.0
-                            : stage_entry(jv, e, rmass[ti] == 0.0, diag, alpha, rmass[ti], alg_only);
+      const double v = stage_entry_select(jv, e, rmass[ti] == 0.0, diag, alpha, rmass[ti], alg_only);
+      a[ti][tj] = (e == -2) ? 1.0 : v;
     }
 }
 
-// Schur update of the core from the sparse pivots: the sums the factor tasks
-// left in LDS are added to the thread's entries.
+// Schur update of the core from the sparse pivots, by the owner of each
+// entry: S = 0 - sum_p L_ip U_pj over the entry's pairs (final sparse
+// entries in LDS).  One flat pass over the thread's own pairs writes its sums
+// in slot order to S[j * NT + t] (pb and the inverse scratch, dead until the
+// core is factored), then each is added to its register.  A thread reads only what
+// it wrote, so no barrier is needed.
 template <int TILE>
-__device__ inline void schur(const PlanDev& P, const double* lds, double (&a)[TILE][TILE]) {
-  const short* sch = P.core_sch + (size_t)threadIdx.x * P.slot_stride;
+__device__ inline void schur(const PlanDev& P, const Tables& T, const double* lds, double* S,
+                             double (&a)[TILE][TILE]) {
+  const int tid = threadIdx.x;
+  const int q1 = P.sch_base[tid + 1];
+  double acc = 0.0;
+  int j = 0;
+  auto step = [&](uint32_t w, double x, double y) {
+    acc = fma(-x, y, acc);
+    if (w >> 31) {
+      S[j * NT + tid] = acc;
+      ++j;
+      acc = 0.0;
+    }
+  };
+  // groups of 4 pairs (16-byte aligned; the zero padding after the last
+  // close only feeds a sum that is never stored)
+  for (int q = P.sch_base[tid]; q < q1; q += 4) {
+    const uint4 w = *reinterpret_cast<const uint4*>(T.sch_pairs + q);
+    const double x0 = lds[(w.x >> 16) & 0x7FFFu], y0 = lds[w.x & 0xFFFFu];
+    const double x1 = lds[(w.y >> 16) & 0x7FFFu], y1 = lds[w.y & 0xFFFFu];
+    const double x2 = lds[(w.z >> 16) & 0x7FFFu], y2 = lds[w.z & 0xFFFFu];
+    const double x3 = lds[(w.w >> 16) & 0x7FFFu], y3 = lds[w.w & 0xFFFFu];
+    step(w.x, x0, y0);
+    step(w.y, x1, y1);
+    step(w.z, x2, y2);
+    step(w.w, x3, y3);
+  }
+  const uint4 mk = *reinterpret_cast<const uint4*>(P.sch_mask + tid * 4);
+  const uint32_t mw[4] = {mk.x, mk.y, mk.z, mk.w};
+  j = 0;
 #pragma unroll
   for (int ti = 0; ti < TILE; ++ti)
 #pragma unroll
     for (int tj = 0; tj < TILE; ++tj) {
-      const int k = sch[ti * TILE + tj];
-      if (k >= 0) a[ti][tj] = a[ti][tj] + lds[P.ns + k];
+      const int s = ti * TILE + tj;
+      const bool has = (mw[s >> 5] >> (s & 31)) & 1u;
+      const double sk = S[j * NT + tid];   // read always (S has sch_depth + 1 rows)
+      a[ti][tj] = has ? a[ti][tj] + sk : a[ti][tj];
+      j += has ? 1 : 0;
     }
 }
 
@@ -335,7 +404,7 @@ __device__ __forceinline__ void factor_cell(const PlanDev& P, const Tables& T, c
                                    double (&a)[TILE][TILE]) {
   __syncthreads();
   run_tasks(T.f_tasks, T.f_ptr, P.f_nst, T.f_pairs, L.lds);
-  schur<TILE>(P, L.lds, a);
+  schur<TILE>(P, T, L.lds, L.pb, a);
   core_factor<TILE>(a, L);
   diag_inverses<TILE>(a, L);
 }

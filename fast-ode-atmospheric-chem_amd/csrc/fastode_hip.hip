@@ -261,7 +261,7 @@ int grid_for(long N, int block = 256) {
 }
 
 size_t engine_bytes(const PlanDev& P) {
-  return (size_t)engine_lds_doubles(P.lds_len, P.mpad, P.tile) * sizeof(double);
+  return (size_t)engine_lds_doubles(P) * sizeof(double);
 }
 
 template <int TILE>
@@ -397,18 +397,30 @@ int fode_plan_create(const fode_plan_desc* d, void** out, int64_t* lu_len) {
   p.n = d->n; p.c = d->c; p.m = d->m; p.tile = d->tile; p.ns = d->ns; p.nsch = d->nsch; p.nnz = d->nnz;
   p.mpad = G * d->tile;
   p.slot_stride = d->slot_stride;
-  p.vec = d->ns + d->nsch;
+  p.vec = d->ns;
   p.lds_len = p.vec + d->c + p.mpad;
   p.lu_len = d->ns + p.mpad * p.mpad + 2 * d->tile * NT;
   p.f_nst = d->f_nst; p.sf_nst = d->sf_nst; p.sb_nst = d->sb_nst;
   p.n_f_tasks = d->n_f_tasks; p.n_f_pairs = d->n_f_pairs; p.n_sf_tasks = d->n_sf_tasks;
-  p.n_sb_tasks = d->n_sb_tasks; p.n_s_pairs = d->n_s_pairs;
+  p.n_sb_tasks = d->n_sb_tasks; p.n_s_pairs = d->n_s_pairs; p.n_sch_pairs = d->n_sch_pairs;
   int rc = 0;
   rc |= upload(d->sp_src, (size_t)d->ns, (int**)&p.sp_src, h->owned);
   rc |= upload(d->sp_diag, (size_t)d->ns, (int**)&p.sp_diag, h->owned);
   rc |= upload(d->sp_row, (size_t)d->ns, (int**)&p.sp_row, h->owned);
   rc |= upload((const short*)d->core_src, (size_t)NT * d->slot_stride, (short**)&p.core_src, h->owned);
-  rc |= upload((const short*)d->core_sch, (size_t)NT * d->slot_stride, (short**)&p.core_sch, h->owned);
+  p.sch_depth = 0;
+  for (int t = 0; t < NT; ++t) {
+    int k = 0;
+    for (int w = 0; w < 4; ++w) k += __builtin_popcount(d->sch_mask[t * 4 + w]);
+    if (k > p.sch_depth) p.sch_depth = k;
+    if (d->sch_base[t] % 4 || d->sch_base[t + 1] < d->sch_base[t]) {
+      delete h;
+      return fail("fode_plan_create: Schur pair ranges must be ordered groups of 4");
+    }
+  }
+  rc |= upload(d->sch_mask, (size_t)NT * 4, (uint32_t**)&p.sch_mask, h->owned);
+  rc |= upload(d->sch_base, (size_t)NT + 1, (int**)&p.sch_base, h->owned);
+  rc |= upload(d->sch_pairs, (size_t)d->n_sch_pairs, (uint32_t**)&p.t.sch_pairs, h->owned);
   rc |= upload(d->core_row, (size_t)p.mpad, (int**)&p.core_row, h->owned);
   rc |= upload((const uint2*)d->f_tasks, (size_t)d->n_f_tasks, (uint2**)&p.t.f_tasks, h->owned);
   rc |= upload(d->f_ptr, (size_t)d->f_nst + 1, (int**)&p.t.f_ptr, h->owned);

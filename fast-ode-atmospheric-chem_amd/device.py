@@ -60,12 +60,12 @@ class DevicePlan:
         plan = build_plan(n, rows, cols, **plan_kw)
         self.plan = plan
         a = plan.arrays
-        unsigned = ("f_tasks", "f_pairs", "sf_tasks", "sb_tasks", "s_pairs")
+        unsigned = ("f_tasks", "f_pairs", "sf_tasks", "sb_tasks", "s_pairs", "sch_pairs", "sch_mask")
         arrs = {}
         for k, v in a.items():
             if k in unsigned:
                 arrs[k] = _u32(v)
-            elif k in ("core_src", "core_sch"):
+            elif k == "core_src":
                 arrs[k] = np.ascontiguousarray(v, dtype=np.int16)
             else:
                 arrs[k] = _i32(v)
@@ -75,7 +75,7 @@ class DevicePlan:
             n_f_tasks=len(a["f_tasks"]) // 2, f_nst=len(a["f_ptr"]) - 1, n_f_pairs=len(a["f_pairs"]),
             n_sf_tasks=len(a["sf_tasks"]) // 2, sf_nst=len(a["sf_ptr"]) - 1,
             n_sb_tasks=len(a["sb_tasks"]) // 2, sb_nst=len(a["sb_ptr"]) - 1,
-            n_s_pairs=len(a["s_pairs"]),
+            n_s_pairs=len(a["s_pairs"]), n_sch_pairs=len(a["sch_pairs"]),
             **{k: _addr(v) for k, v in arrs.items()})
         h = ct.c_void_p()
         lu_len = ct.c_int64()

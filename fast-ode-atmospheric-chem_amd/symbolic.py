@@ -18,14 +18,13 @@ pattern fixed here once per mechanism:
    dot-product tasks  A_ij = (A_ij - sum_{p < min(i,j)} L_ip U_pj) [/ U_jj],
    grouped in stages: a task runs once every task it reads has run;
 5. the core's Schur complement update from the sparse pivots,
-   S_ij = -sum_{p < c} L_ip U_pj, is one more dot-product task per core entry
-   that has one, writing S_ij to its own LDS slot; each thread then adds the
-   sums of its entries to its register tile;
+   S_ij = -sum_{p < c} L_ip U_pj, is formed by the thread that owns the core
+   entry, straight into its register tile, from its own list of pairs;
 6. the solves use level-scheduled row tasks for the sparse rows and block
    substitution with inverted diagonal blocks for the core.
 
-LDS index space of one cell: [0, ns) sparse entries, [ns, ns + nsch) Schur
-sums, then the solve vector (c + m_pad entries).
+LDS index space of one cell: [0, ns) sparse entries, then the solve vector
+(c + m_pad entries).
 
 Table formats (all little integers, so that they fit in LDS):
 * task  = 2 x uint32: [target | divisor << 16] (divisor 0xFFFF: none) and
@@ -33,7 +32,9 @@ Table formats (all little integers, so that they fit in LDS):
 * pair  = uint32 a << 16 | b  (LDS indices of the two factors);
 * core_src = int16 [thread][slot_stride]: Jacobian entry, -1 fill, -2 the
   identity diagonal of a padding row;
-* core_sch = int16 [thread][slot_stride]: Schur sum of the entry, -1 none.
+* sch_mask = uint32 [thread][4]: bit s set when core slot s has a Schur sum;
+  the thread's pairs contiguous in sch_pairs from sch_base[thread], bit 31
+  of the pair that closes a sum.
 """
 from __future__ import annotations
 
@@ -147,11 +148,17 @@ class LUPlan:
 
     @property
     def lds_vec(self):
-        return self.ns + self.nsch
+        return self.ns
 
     @property
     def lds_len(self):
-        return self.ns + self.nsch + self.c + self.m_pad
+        return self.ns + self.c + self.m_pad
+
+    @property
+    def sch_depth(self):
+        """Most Schur sums one thread holds."""
+        m = self.arrays["sch_mask"].reshape(NT, 4)
+        return int(max(sum(bin(int(x)).count("1") for x in row) for row in m))
 
     @property
     def lu_len(self):
@@ -163,7 +170,7 @@ class LUPlan:
         """uint32 words of the tables a persistent workgroup keeps in LDS."""
         a = self.arrays
         return sum(len(a[k]) for k in ("f_tasks", "f_ptr", "f_pairs", "sf_tasks", "sf_ptr", "sb_tasks",
-                                        "sb_ptr", "s_pairs"))
+                                        "sb_ptr", "s_pairs", "sch_pairs"))
 
     def flops(self):
         """Multiply-adds (+ divisions) of one factorization and the L+U entry
@@ -222,7 +229,7 @@ def build_plan(n, rows, cols, tile=None, rho=DENSE_RHO, order=None, m_max=None):
             if ps:
                 sch_entries.append((ci, cj, [(int(sp_index[i, p]), int(sp_index[p, j])) for p in ps]))
     nsch = len(sch_entries)
-    VEC = ns + nsch
+    VEC = ns
     if VEC + c + m_pad >= NO_DIV:
         raise ValueError("the LDS index space of one cell exceeds 16 bits")
 
@@ -250,11 +257,33 @@ def build_plan(n, rows, cols, tile=None, rho=DENSE_RHO, order=None, m_max=None):
                             core_src[t, s] = -2
                         continue
                     core_src[t, s] = jac_of[c + ci, c + cj]
-    core_sch = np.full((NT, stride), -1, dtype=np.int16)
-    for k, (ci, cj, _) in enumerate(sch_entries):
+    # Schur sums S = 0 - sum_p L_ip U_pj of the core entries, formed by the
+    # thread that owns the entry in one pass over its own pairs: its sums in
+    # slot order, pairs from sch_base[t] to sch_base[t + 1] (padded with 0 to
+    # groups of 4 after the last sum), bit 31 of a pair closing a sum;
+    # sch_mask[t] has bit s set when slot s has a sum.
+    by_slot = {}
+    for ci, cj, prs in sch_entries:
         ti, tr = divmod(ci, G)
         tj, tc = divmod(cj, G)
-        core_sch[tr * G + tc, ti * tile + tj] = k
+        by_slot[(tr * G + tc, ti * tile + tj)] = prs
+    if by_slot and VEC + c + m_pad >= 1 << 15:
+        raise ValueError("Schur pairs need LDS indices below 32768")
+    sch_mask = np.zeros((NT, 4), dtype=np.uint32)
+    sch_base = np.zeros(NT + 1, dtype=np.int32)
+    sch_words = []
+    for t in range(NT):
+        sch_base[t] = len(sch_words)
+        for slot in range(tile * tile):
+            prs = by_slot.get((t, slot))
+            if prs:
+                sch_mask[t, slot >> 5] |= np.uint32(1 << (slot & 31))
+                for q, (a_, b_) in enumerate(prs):
+                    sch_words.append((int(a_) << 16) | int(b_) | ((1 << 31) if q == len(prs) - 1 else 0))
+        while len(sch_words) % 4:
+            sch_words.append(0)
+    sch_base[NT] = len(sch_words)
+    sch_pairs = np.asarray(sch_words, dtype=np.uint32)
 
     # ---- sparse factorization tasks (dot-product form)
     tgt, pstart, pcount, div, level = [], [], [], [], []
@@ -283,19 +312,6 @@ def build_plan(n, rows, cols, tile=None, rho=DENSE_RHO, order=None, m_max=None):
         pstart.append(start)
         pcount.append(len(ps))
         div.append(d)
-        level.append(L - 1)
-    # Schur sums S = 0 - sum_p L_ip U_pj (the slot starts at zero)
-    for k, (_, _, prs) in enumerate(sch_entries):
-        start = len(pairs_a)
-        L = 0
-        for a, b in prs:
-            pairs_a.append(a)
-            pairs_b.append(b)
-            L = max(L, lev[a] + 1, lev[b] + 1)
-        tgt.append(ns + k)
-        pstart.append(start)
-        pcount.append(len(prs))
-        div.append(-1)
         level.append(L - 1)
     fo, fptr = _stage(level)
     f_tasks = _pack_tasks(*(np.asarray(x)[fo] if len(tgt) else [] for x in (tgt, pstart, pcount, div)))
@@ -346,7 +362,8 @@ def build_plan(n, rows, cols, tile=None, rho=DENSE_RHO, order=None, m_max=None):
 
     arrays = dict(
         sp_src=sp_src, sp_diag=sp_diag, sp_row=sp_row,
-        core_src=core_src.ravel(), core_sch=core_sch.ravel(), core_row=core_row,
+        core_src=core_src.ravel(), sch_mask=sch_mask.ravel(), sch_base=sch_base, sch_pairs=sch_pairs,
+        core_row=core_row,
         f_tasks=f_tasks, f_ptr=fptr.astype(np.int32), f_pairs=f_pairs,
         sf_tasks=sf_tasks, sf_ptr=sfptr.astype(np.int32),
         sb_tasks=sb_tasks, sb_ptr=sbptr.astype(np.int32),
@@ -393,15 +410,22 @@ def emulate_factor(plan: LUPlan, A_old: np.ndarray):
     _run_tasks(lds, ar["f_tasks"], ar["f_ptr"], ar["f_pairs"])
     core = np.eye(mp)
     core[:m, :m] = A[c:, c:] * plan.F[c:, c:]
-    sch = ar["core_sch"].reshape(NT, plan.slot_stride)
+    words = ar["sch_pairs"].astype(np.int64)
+    mask = ar["sch_mask"].reshape(NT, 4).astype(np.int64)
     tile = plan.tile
     for t in range(NT):
+        sums, acc = [], 0.0
+        for q in range(int(ar["sch_base"][t]), int(ar["sch_base"][t + 1])):
+            w = words[q]
+            acc -= lds[(w >> 16) & 0x7FFF] * lds[w & 0xFFFF]
+            if w >> 31:
+                sums.append(acc)
+                acc = 0.0
         tr, tc = divmod(t, G)
-        for s in range(tile * tile):
-            k = int(sch[t, s])
-            if k >= 0:
-                ti, tj = divmod(s, tile)
-                core[ti * G + tr, tj * G + tc] += lds[plan.ns + k]
+        for s_ in range(tile * tile):
+            if (mask[t, s_ >> 5] >> (s_ & 31)) & 1:
+                ti, tj = divmod(s_, tile)
+                core[ti * G + tr, tj * G + tc] += sums.pop(0)
     for k in range(mp):
         core[k + 1:, k] /= core[k, k]
         core[k + 1:, k + 1:] -= np.outer(core[k + 1:, k], core[k, k + 1:])

@@ -84,10 +84,12 @@ typedef struct fode_mech_desc {
 typedef struct fode_plan_desc {
   int32_t n, c, m, tile, ns, nsch, nnz, slot_stride;
   int32_t n_f_tasks, f_nst, n_f_pairs;
-  int32_t n_sf_tasks, sf_nst, n_sb_tasks, sb_nst, n_s_pairs;
+  int32_t n_sf_tasks, sf_nst, n_sb_tasks, sb_nst, n_s_pairs, n_sch_pairs;
   const int32_t* sp_src; const int32_t* sp_diag; const int32_t* sp_row;     /* ns */
-  const int16_t* core_src;   /* 256 * slot_stride */
-  const int16_t* core_sch;   /* 256 * slot_stride */
+  const int16_t* core_src;   /* 256 x slot_stride, thread-major */
+  const uint32_t* sch_mask;  /* 256 x 4: bit s set when core slot s has a Schur sum */
+  const int32_t* sch_base;   /* 257: each thread's Schur pairs [sch_base[t], sch_base[t+1]), a multiple of 4 */
+  const uint32_t* sch_pairs; /* n_sch_pairs: a << 16 | b, bit 31 on the pair closing a sum */
   const int32_t* core_row;   /* 16 * tile */
   const uint32_t* f_tasks; const int32_t* f_ptr; const uint32_t* f_pairs;
   const uint32_t* sf_tasks; const int32_t* sf_ptr;
@@ -184,7 +186,8 @@ typedef struct fode_batch_desc {
    * overflows). */
   double* hlog;
   int32_t* nacc;
-  int32_t hcap, pad_;
+  int32_t hcap;
+  int32_t options;         /* bit 0: keep the plan's tables in memory (no LDS staging) */
 } fode_batch_desc;
 
 /* Launch one interval. blocks_per_cu <= 0 picks the occupancy limit. */

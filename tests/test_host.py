@@ -72,10 +72,18 @@ def test_plan_fill_is_a_superset_and_levels_are_causal():
                 w = int(a["f_pairs"][q])
                 for src in (w >> 16, w & 0xFFFF):
                     assert done_stage.get(src, -1) < s
-    # every Schur sum is used by exactly one core slot
-    sch = a["core_sch"]
-    used = np.sort(sch[sch >= 0])
-    assert np.array_equal(used, np.arange(plan.nsch))
+    # Schur sums: each used by exactly one core slot, closed by exactly one
+    # pair of its owner's list, whose factors are sparse entries (final after
+    # the factor tasks, which the Schur update follows)
+    mask = a["sch_mask"].reshape(256, 4)
+    per_thread = np.array([sum(bin(int(x)).count("1") for x in row) for row in mask])
+    assert per_thread.sum() == plan.nsch and per_thread.max() == plan.sch_depth
+    base = a["sch_base"].astype(np.int64)
+    assert base[0] == 0 and base[-1] == len(a["sch_pairs"]) and (base % 4 == 0).all()
+    pr = a["sch_pairs"].astype(np.int64)
+    for t in range(256):
+        assert int((pr[base[t]:base[t + 1]] >> 31).sum()) == per_thread[t]
+    assert (((pr >> 16) & 0x7FFF) < plan.ns).all() and ((pr & 0xFFFF) < plan.ns).all()
 
 
 def test_mechanism_tables_match_the_oracle():
