@@ -95,7 +95,7 @@ def cpu_baseline(C, n_groups, n_int):
                       f"intervals = {done} cell-intervals in {wall:.1f} s, spawn pool, 1 BLAS thread each"}
 
 
-def adjoint_leg(fa, torch, W, C, n_int):
+def adjoint_leg(fa, torch, W, C, n_int, choice=None):
     """Gradient of sum_c sum_s w_s log(Y_final + 1e-17) with respect to the
     log rate constants (C, R) and Y0, over n_int intervals: time of the plain
     batched forward (graph replays) against the taped forward (checkpoints,
@@ -108,6 +108,9 @@ def adjoint_leg(fa, torch, W, C, n_int):
     kmuls = torch.as_tensor(wl.kmuls(), device=dev)
     Y0 = torch.as_tensor(wl.Y0, device=dev)
     w = torch.as_tensor(wl.weights, device=dev)
+    if choice is not None:      # the plan the tuner picked for this mechanism
+        from fastode_amd.autotune import install
+        install(wl.mech, choice)
     adj = BatchedAdjoint(wl.mech)
     # warm: one interval of a small batch through both passes
     Yw, tw = adj.forward(k[:64], Y0[:64], kmuls[:1], wl.dt, wl.atol, wl.rtol)
@@ -153,6 +156,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--blocks-per-cu", type=int, default=0)
+    ap.add_argument("--no-tune", action="store_true", help="keep the default plan")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="first run a plain copy kernel over this many MiB (HBM counter calibration)")
     ap.add_argument("--adjoint-cells", type=int, default=16384, help="0 skips the adjoint leg")
     ap.add_argument("--adjoint-intervals", type=int, default=24)
     args = ap.parse_args()
@@ -171,6 +177,15 @@ def main():
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
+    if args.copy_probe_mib > 0:
+        # an elementwise kernel that reads and writes N MiB once each
+        xs = torch.ones(args.copy_probe_mib << 17, dtype=torch.float64, device=dev)
+        ys = torch.empty_like(xs)
+        for _ in range(3):
+            torch.mul(xs, 1.0, out=ys)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": xs.numel() * 8, "copy_probe_bytes_written": ys.numel() * 8}))
+        del xs, ys
     wl = W.diurnal200(C=args.cells, n_intervals=args.steps)
     mech = wl.mech
     K = args.steps
@@ -180,6 +195,16 @@ def main():
         m[wl.photolysis] = diurnal_factor(it * wl.dt)
         return m
 
+    tuned = None
+    if not args.no_tune:
+        # autotuner: one interval of the first 8,192 cells per candidate plan
+        from fastode_amd.autotune import tune
+        n_s = min(wl.C, 8192)
+        t_tune = time.perf_counter()
+        ch = tune(mech, wl.k_base[None, :], wl.Y0[:n_s], wl.dt, wl.atol, wl.rtol, kmul=kmul(0), use_cache=False)
+        tuned = {"rho": ch.rho, "m_max": ch.m_max, "stage_tables": ch.stage_tables, "c": ch.c, "tile": ch.tile,
+                 "us_per_cell_attempt": round(ch.us_per_cell_attempt, 3),
+                 "tuning_s": round(time.perf_counter() - t_tune, 1)}
     system = ChemSystem(mech, wl.k_base[None, :])
     balg = fa.BatchedAlgorithm(fa.ROS3, blocks_per_cu=args.blocks_per_cu)
     plan = mechanism_plan(mech).plan
@@ -220,11 +245,17 @@ def main():
     alg_flops = algorithmic_flops(counts, cells_per_group, wl.n, wl.R, mech.P, mech.nnz, fact_flops, nnz_lu,
                                   len(mech.jac_src), len(mech.rhs_rxn))
     achieved = alg_bytes / kernel_s / 1e9
+    # HBM traffic per launch: measured bytes per cell-attempt (rocprofv3 PMC
+    # FETCH_SIZE + WRITE_SIZE of this kernel, scaled by a copy kernel's
+    # reading) times this run's cell-attempts per launch
     traffic = None
     pmc = ROOT / "profiles" / "traffic_mech200-diurnal.json"
     if pmc.exists():
         t = json.loads(pmc.read_text())
-        traffic = t.get("hbm_bytes_per_launch")
+        per_attempt = t.get("hbm_bytes_per_cell_attempt")
+        if per_attempt:
+            cell_attempts = float((counts[:, 0] * cells_per_group).sum()) / K
+            traffic = round(per_attempt * cell_attempts)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "fp64_tflops": round(alg_flops / kernel_s / 1e12, 3), "fp64_frac": round(
@@ -239,7 +270,8 @@ def main():
                       "group_size": 32, "groups": run.ngroups, "intervals": K, "dt_s": wl.dt,
                       "rtol": wl.rtol, "atol": float(wl.atol[0]), "tableau": "ros3",
                       "form": "batched persistent kernel, one hipGraph replay per interval",
-                      "lu": {"nnz_J": mech.nnz, "nnz_LU": nnz_lu, "core": plan.m, "tile": plan.tile}},
+                      "lu": {"nnz_J": mech.nnz, "nnz_LU": nnz_lu, "core": plan.m, "tile": plan.tile},
+                      "autotuned": tuned},
            "attempts_per_group_interval": round(float(counts[:, 0].sum()) / (run.ngroups * K), 3),
            "rejected_per_group_interval": round(float(counts[:, 2].sum()) / (run.ngroups * K), 3),
            "failed_groups": fails,
@@ -273,7 +305,8 @@ def main():
         out["speedup_vs_plain"] = round(value / plain_value, 1)
 
     if args.adjoint_cells > 0:
-        out["adjoint"] = adjoint_leg(fa, torch, W, args.adjoint_cells, args.adjoint_intervals)
+        out["adjoint"] = adjoint_leg(fa, torch, W, args.adjoint_cells, args.adjoint_intervals,
+                                     getattr(mech, "_tuned", None))
         out["adjoint_over_forward"] = out["adjoint"]["ratio_total"]
 
     if not args.no_cpu:

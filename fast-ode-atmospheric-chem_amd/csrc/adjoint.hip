@@ -268,7 +268,8 @@ int device_limits(int* cus, int* maxlds) {
 int check_pair(const PlanDev& P, const PlanDev& PT, const MechDev& M) {
   if (P.n != M.n || PT.n != M.n || P.nnz != M.nnz || PT.nnz != M.nnz)
     return afail("fode_adjoint: plans and mechanism differ");
-  if (P.c != PT.c || P.tile != PT.tile || P.lds_len != PT.lds_len || P.mpad != PT.mpad || P.ns != PT.ns)
+  if (P.c != PT.c || P.tile != PT.tile || P.lds_len != PT.lds_len || P.mpad != PT.mpad || P.ns != PT.ns ||
+      engine_lds_doubles(P) != engine_lds_doubles(PT))
     return afail("fode_adjoint: the transposed plan's split differs from the plan's");
   if (M.ncon > 0) return afail("fode_adjoint: constrained systems are not supported");
   return 0;

@@ -59,6 +59,7 @@ class DevicePlan:
     def __init__(self, n, rows, cols, **plan_kw):
         plan = build_plan(n, rows, cols, **plan_kw)
         self.plan = plan
+        self.plan_kw = dict(plan_kw)
         a = plan.arrays
         unsigned = ("f_tasks", "f_pairs", "sf_tasks", "sb_tasks", "s_pairs", "sch_pairs", "sch_mask")
         arrs = {}
@@ -114,8 +115,10 @@ def transposed_plan(mech) -> DevicePlan:
     forward plan's (the no-pivot fill of A^T is the transpose of A's)."""
     dp = getattr(mech, "_plan_t", None)
     if dp is None:
-        fwd = mechanism_plan(mech).plan
-        dp = DevicePlan(mech.n, mech.cols, mech.rows, order=fwd.perm, tile=fwd.tile)
+        fp = mechanism_plan(mech)
+        fwd = fp.plan
+        kw = {k: v for k, v in fp.plan_kw.items() if k in ("rho", "m_max")}
+        dp = DevicePlan(mech.n, mech.cols, mech.rows, order=fwd.perm, tile=fwd.tile, **kw)
         t = dp.plan
         if (t.c, t.m, t.tile, t.ns, t.nsch) != (fwd.c, fwd.m, fwd.tile, fwd.ns, fwd.nsch):
             raise RuntimeError("the transposed plan's split differs from the forward plan's")
