@@ -54,7 +54,7 @@ AdjArgs make_args(const PlanDev& P, const PlanDev& PT, const MechDev& M, int S) 
   A.lay = make_layout(P, M, S, false);
   A.adj = A.lay.total;
   int end = A.adj + 2 * M.n + 2 * S * M.n + M.R;
-  const int scr_at = A.lay.engine + (P.lds_len + 4 * P.mpad + 2 + 2 * NT);
+  const int scr_at = A.lay.engine + engine_scr_offset(P);
   if (M.R * M.P + M.R <= P.tile * NT) {   // the inverse scratch is dead after the factorization
     A.phi = scr_at;
     A.gr = scr_at + M.R * M.P;

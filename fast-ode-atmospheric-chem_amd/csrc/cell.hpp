@@ -36,7 +36,7 @@ inline Layout make_layout(const PlanDev& P, const MechDev& M, int S, bool stage)
   // only the inverse scratch (scr) is dead; they get a region of their own if
   // scr is too small.
   const int scr = P.tile * NT;
-  const int scr_at = l.engine + (P.lds_len + 4 * P.mpad + 2 + 2 * NT);
+  const int scr_at = l.engine + engine_scr_offset(P);
   const int need_rd = M.R * M.P > M.R ? M.R * M.P : M.R;
   if (need_rd <= scr) {
     l.rd = scr_at;

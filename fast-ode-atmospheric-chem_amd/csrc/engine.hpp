@@ -23,7 +23,7 @@ struct EngineLds {
   double* lds;    // [lds_len]: sparse entries, then the solve vector (c + mpad)
   double* lbuf;   // [2*mpad]   pivot column, double-buffered
   double* ubuf;   // [2*mpad]   pivot row
-  double* rbuf;   // [2]        pivot reciprocal
+  double* rbuf;   // [mpad]     pivot reciprocals
   double* pb;     // [2*NT]     block-solve partial products
   double* scr;    // [tile*NT]  diagonal blocks being inverted
   double* dL;     // [tile*NT]  inverse diagonal blocks of L, transposed
@@ -34,10 +34,13 @@ struct EngineLds {
 // (while the Schur update runs, before any of these is live; the pivot
 // buffers are written as soon as a thread leaves it) the Schur sums,
 // thread-interleaved S[j * NT + t] from pb on.
+__host__ __device__ inline int engine_pb_offset(const PlanDev& P) { return P.lds_len + 5 * P.mpad; }
+__host__ __device__ inline int engine_scr_offset(const PlanDev& P) { return engine_pb_offset(P) + 2 * NT; }
+
 __host__ __device__ inline int engine_lds_doubles(const PlanDev& P) {
   const int rest = 2 * NT + 3 * P.tile * NT;
   const int sums = (P.sch_depth + 1) * NT;
-  return P.lds_len + 4 * P.mpad + 2 + (rest > sums ? rest : sums);
+  return engine_pb_offset(P) + (rest > sums ? rest : sums);
 }
 
 __device__ inline EngineLds carve(double* base, const PlanDev& P) {
@@ -46,7 +49,7 @@ __device__ inline EngineLds carve(double* base, const PlanDev& P) {
   e.lbuf = e.lds + P.lds_len;
   e.ubuf = e.lbuf + 2 * P.mpad;
   e.rbuf = e.ubuf + 2 * P.mpad;
-  e.pb = e.rbuf + 2;
+  e.pb = e.rbuf + P.mpad;
   e.scr = e.pb + 2 * NT;
   e.dL = e.scr + P.tile * NT;
   e.dU = e.dL + P.tile * NT;
@@ -247,10 +250,11 @@ __device__ inline void schur(const PlanDev& P, const Tables& T, const double* ld
     }
 }
 
-// Publish pivot k1 = NB*G + kk1 (column, row, reciprocal pivot) into buffer nb.
+// Publish pivot k1 = NB*G + kk1 (raw column, row, and the pivot's
+// reciprocal at rcs[k1]) into buffer nb.
 template <int TILE, int NB>
 __device__ inline void publish(const double (&a)[TILE][TILE], int kk1, int nb, double* lbuf,
-                               double* ubuf, double* rbuf, int tr, int tc) {
+                               double* ubuf, double* rcs, int tr, int tc) {
   constexpr int MP = G * TILE;
   if (tc == kk1) {
 #pragma unroll
@@ -259,39 +263,53 @@ __device__ inline void publish(const double (&a)[TILE][TILE], int kk1, int nb, d
   if (tr == kk1) {
 #pragma unroll
     for (int tj = NB; tj < TILE; ++tj) ubuf[nb * MP + tj * G + tc] = a[NB][tj];
-    if (tc == kk1) rbuf[nb] = 1.0 / a[NB][NB];
+    if (tc == kk1) rcs[NB * G + kk1] = 1.0 / a[NB][NB];
   }
 }
 
+// The 16 pivots of tile block B.  Only tile row/column B straddle the
+// pivot, so only there are the multipliers and pivot-row values masked;
+// the owners of the pivot columns keep their raw entries (the update leaves
+// them alone: their u is 0) and scale them by the pivots' reciprocals once,
+// after the block -- the same product l = raw * (1/pivot) the consumers use.
 template <int TILE, int B>
 __device__ inline void factor_block(double (&a)[TILE][TILE], double* lbuf, double* ubuf,
-                                    double* rbuf, int tr, int tc) {
+                                    double* rcs, int tr, int tc) {
   constexpr int MP = G * TILE;
 #pragma nounroll
   for (int kk = 0; kk < G; ++kk) {
     const int k = B * G + kk;
     const int buf = k & 1;
-    const double rc = rbuf[buf];
+    const double rc = rcs[k];
     const double* lb = lbuf + buf * MP + tr;
     const double* ub = ubuf + buf * MP + tc;
     double u[TILE];
+    u[B] = (tc > kk) ? ub[B * G] : 0.0;
 #pragma unroll
-    for (int tj = B; tj < TILE; ++tj) u[tj] = (tj * G + tc > k) ? ub[tj * G] : 0.0;
-    const bool owner = (tc == kk);
+    for (int tj = B + 1; tj < TILE; ++tj) u[tj] = ub[tj * G];
 #pragma unroll
     for (int ti = B; ti < TILE; ++ti) {
-      const bool below = ti * G + tr > k;
-      const double l = below ? lb[ti * G] * rc : 0.0;
-      if (owner && below) a[ti][B] = l;
+      double l = lb[ti * G] * rc;
+      if (ti == B) l = (tr > kk) ? l : 0.0;
 #pragma unroll
       for (int tj = B; tj < TILE; ++tj) a[ti][tj] = fma(-l, u[tj], a[ti][tj]);
     }
     if (kk + 1 < G) {
-      publish<TILE, B>(a, kk + 1, buf ^ 1, lbuf, ubuf, rbuf, tr, tc);
+      publish<TILE, B>(a, kk + 1, buf ^ 1, lbuf, ubuf, rcs, tr, tc);
     } else if constexpr (B + 1 < TILE) {
-      publish<TILE, B + 1>(a, 0, buf ^ 1, lbuf, ubuf, rbuf, tr, tc);
+      publish<TILE, B + 1>(a, 0, buf ^ 1, lbuf, ubuf, rcs, tr, tc);
     }
     __syncthreads();
+  }
+  // the L entries of tile column B (below the pivots' diagonal)
+  const double rcb = rcs[B * G + tc];
+#pragma unroll
+  for (int ti = B; ti < TILE; ++ti) {
+    if (ti == B) {
+      if (tr > tc) a[ti][B] = a[ti][B] * rcb;
+    } else {
+      a[ti][B] = a[ti][B] * rcb;
+    }
   }
 }
 
@@ -354,6 +372,18 @@ __device__ inline void diag_inverses(const double (&a)[TILE][TILE], const Engine
   }
 }
 
+// Sum of p[q*G] for q < 16 as a tree (4 dependent adds, not 15).
+__device__ inline double sum16(const double* p) {
+  double t[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) t[q] = p[q * G] + p[(q + 8) * G];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) t[q] = t[q] + t[q + 4];
+  t[0] = t[0] + t[2];
+  t[1] = t[1] + t[3];
+  return t[0] + t[1];
+}
+
 // Block forward/back substitution of the core: v (length mpad) in LDS holds
 // the right-hand side on entry and the solution on exit.
 template <int TILE>
@@ -370,9 +400,7 @@ __device__ inline void core_solve(const double (&a)[TILE][TILE], const EngineLds
     double* p = L.pb + (J & 1) * NT;
     p[tid] = d * r;
     __syncthreads();
-    double y = 0.0;
-#pragma unroll
-    for (int q = 0; q < G; ++q) y += p[q * G + tc];
+    const double y = sum16(p + tc);
     if (tr == 0) v[J * G + tc] = y;
 #pragma unroll
     for (int I = J + 1; I < TILE; ++I) acc[I] = fma(a[I][J], y, acc[I]);
@@ -388,9 +416,7 @@ __device__ inline void core_solve(const double (&a)[TILE][TILE], const EngineLds
     double* p = L.pb + (J & 1) * NT;
     p[tid] = d * r;
     __syncthreads();
-    double x = 0.0;
-#pragma unroll
-    for (int q = 0; q < G; ++q) x += p[q * G + tc];
+    const double x = sum16(p + tc);
     if (tr == 0) v[J * G + tc] = x;
 #pragma unroll
     for (int I = 0; I < J; ++I) acc[I] = fma(a[I][J], x, acc[I]);
