@@ -157,6 +157,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--no-tune", action="store_true", help="keep the default plan")
+    ap.add_argument("--no-order", action="store_true", help="take groups in index order (not costliest first)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run a plain copy kernel over this many MiB (HBM counter calibration)")
     ap.add_argument("--adjoint-cells", type=int, default=16384, help="0 skips the adjoint leg")
@@ -206,7 +207,7 @@ def main():
                  "us_per_cell_attempt": round(ch.us_per_cell_attempt, 3),
                  "tuning_s": round(time.perf_counter() - t_tune, 1)}
     system = ChemSystem(mech, wl.k_base[None, :])
-    balg = fa.BatchedAlgorithm(fa.ROS3, blocks_per_cu=args.blocks_per_cu)
+    balg = fa.BatchedAlgorithm(fa.ROS3, blocks_per_cu=args.blocks_per_cu, longest_first=not args.no_order)
     plan = mechanism_plan(mech).plan
 
     # warmup on a copy of the batch

@@ -42,6 +42,7 @@ CTRL_DTYPE = np.dtype([
 assert CTRL_DTYPE.itemsize == 240
 CTRL_WORDS = CTRL_DTYPE.itemsize // 8
 ALPHA_OFFSET = CTRL_DTYPE.fields["alpha"][1]
+NSTEPS_WORD = CTRL_DTYPE.fields["nsteps"][1] // 4
 
 
 class CtrlParams(ct.Structure):
@@ -84,7 +85,7 @@ class BatchDesc(ct.Structure):
                 ("Y", ct.c_void_p), ("k", ct.c_void_p), ("kmul", ct.c_void_p), ("atol", ct.c_void_p),
                 ("mass", ct.c_void_p), ("g", ct.c_void_p), ("work", ct.c_void_p),
                 ("queue", ct.c_void_p), ("hlog", ct.c_void_p), ("nacc", ct.c_void_p),
-                ("hcap", ct.c_int32), ("options", ct.c_int32)]
+                ("hcap", ct.c_int32), ("options", ct.c_int32), ("order", ct.c_void_p)]
 
 
 class AdjointDesc(ct.Structure):

@@ -48,7 +48,8 @@ class BatchedAlgorithm:
 
     name = "batched-algorithm"
 
-    def __init__(self, tableau=ROS3, group_size=32, blocks_per_cu=0, stage_tables=None, **options):
+    def __init__(self, tableau=ROS3, group_size=32, blocks_per_cu=0, stage_tables=None, longest_first=True,
+                 **options):
         """stage_tables: copy the plan's tables into LDS when they fit (True),
         never (False), or as the mechanism's tuned choice says (None; stage
         when untuned)."""
@@ -57,6 +58,7 @@ class BatchedAlgorithm:
         self.group_size = group_size
         self.blocks_per_cu = blocks_per_cu
         self.stage_tables = stage_tables
+        self.longest_first = longest_first
 
     def new_memory(self, C):
         return GroupMemories(-(-C // self.group_size))
@@ -106,6 +108,9 @@ class BatchedRunner:
         # ynew per cell; the projection's corrections and saved state when constrained
         self.work = torch.empty(((2 if system.has_constraints else 1) * C, n), dtype=F64, device=self.Y.device)
         self.queue = torch.zeros(1, dtype=torch.int32, device=self.Y.device)
+        # order the workgroups take groups in: by the previous call's attempts,
+        # most first (no effect on results)
+        self.order = torch.arange(self.ngroups, dtype=torch.int32, device=self.Y.device)
         self.params = ctrl_params(balg.alg, balg.tableau, system.has_constraints)
         self.plan = mechanism_plan(system.mech)
         self.dmech = device_mechanism(system.mech)
@@ -114,7 +119,8 @@ class BatchedRunner:
             dt=float(dt), rtol=float(rtol), Y=self.Y.data_ptr(), k=k.data_ptr(),
             kmul=self.kmul.data_ptr(), atol=self.atol.data_ptr(), mass=system.mass_dev.data_ptr(),
             g=self.memory.dev.data_ptr(), work=self.work.data_ptr(), queue=self.queue.data_ptr(),
-            options=0 if self._stage(balg, system) else 1)
+            options=0 if self._stage(balg, system) else 1,
+            order=self.order.data_ptr() if balg.longest_first else None)
         self.hcap = int(balg.alg.max_steps)
         if record:
             # accepted steps of the last call: H per group, and their number
@@ -174,6 +180,11 @@ class BatchedRunner:
             L.check(L.lib.fode_batched_interval(self.plan.handle, self.dmech.handle, ct.byref(self.params),
                                                 ct.byref(self.desc), int(self.balg.blocks_per_cu), s),
                     "fode_batched_interval")
+        if self.balg.longest_first:
+            # attempts of this call per group (fode_group_ctrl.nsteps), most first
+            nsteps = self.memory.dev.view(torch.int32).view(self.ngroups, L.CTRL_DTYPE.itemsize // 4)[
+                :, L.NSTEPS_WORD]
+            self.order.copy_(torch.argsort(nsteps, descending=True, stable=True).to(torch.int32))
 
     def lds_bytes(self):
         b = ct.c_int64()

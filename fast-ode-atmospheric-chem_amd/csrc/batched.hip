@@ -268,7 +268,15 @@ __global__ __launch_bounds__(NT) void k_batched(BatchArgs A) {
   const int tid = threadIdx.x;
   const fode_batch_desc& B = A.b;
   for (;;) {
-    if (tid == 0) V.flags[0] = atomicAdd(B.queue, 1);
+    if (tid == 0) {
+      const int idx = atomicAdd(B.queue, 1);
+      int grp = idx;
+      if (idx < B.ngroups && B.order) {
+        grp = B.order[idx];
+        if (grp < 0 || grp >= B.ngroups) grp = idx;   // not a permutation: ignore it
+      }
+      V.flags[0] = grp;
+    }
     __syncthreads();
     const int group = V.flags[0];
     __syncthreads();

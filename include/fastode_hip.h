@@ -188,6 +188,10 @@ typedef struct fode_batch_desc {
   int32_t* nacc;
   int32_t hcap;
   int32_t options;         /* bit 0: keep the plan's tables in memory (no LDS staging) */
+  /* Optional order in which the persistent workgroups take groups (a
+   * permutation of 0..ngroups-1), or NULL for 0, 1, 2, ...: costliest first
+   * shortens the tail of an interval.  Results do not depend on it. */
+  const int32_t* order;
 } fode_batch_desc;
 
 /* Launch one interval. blocks_per_cu <= 0 picks the occupancy limit. */
