@@ -66,21 +66,33 @@ def _oracle_job(args):
     import __graft_entry__
     __graft_entry__.load_package()
     from fastode_amd import workloads as W
-    C, group, n_int = args
-    wl = W.diurnal200(C=C)
-    mech = wl.mech
-    om = O.OracleMechanism(mech.species, mech.reactions_list())
+    name, C, group, n_int = args
     cells = np.arange(group * 32, group * 32 + 32)
-    alg = O.Algorithm(O.ROS3)
+    if name == "diurnal":
+        wl = W.diurnal200(C=C)
+    elif name == "uneven":
+        wl = W.uneven200(cells=cells)
+        cells = np.arange(32)
+    else:
+        wl = W.constrained100(C=C)
+    mech = wl.mech
+    cons = None
+    if len(mech.constraint_rows):
+        cons = {mech.species[r]: ({mech.species[s]: float(mech.constraint_coef[i, s])
+                                   for s in np.nonzero(mech.constraint_coef[i])[0]},
+                                  float(mech.constraint_total[i]))
+                for i, r in enumerate(mech.constraint_rows)}
+    om = O.OracleMechanism(mech.species, mech.reactions_list(), cons)
+    alg = O.Algorithm(O.ROS3 if wl.tableau == "ros3" else O.RODAS3)
     mem = alg.new_memory()
     Y = wl.Y0[cells].copy()
     for it in range(n_int):
-        Y, _ = alg.solve(O.ChemSystem(om, wl.k(it, cells), dense=False), Y, wl.dt, wl.atol, wl.rtol,
-                         O.Counts(), mem)
+        Y, _ = alg.solve(O.ChemSystem(om, wl.k(it, cells), mass=wl.mass, dense=False), Y, wl.dt, wl.atol,
+                         wl.rtol, O.Counts(), mem)
     return 32 * n_int
 
 
-def cpu_baseline(C, n_groups, n_int):
+def cpu_baseline(name, C, n_groups, n_int):
     import multiprocessing as mp
     cores = max(1, min(16, len(os.sched_getaffinity(0))))
     os.environ["OMP_NUM_THREADS"] = "1"
@@ -88,7 +100,7 @@ def cpu_baseline(C, n_groups, n_int):
     ctx = mp.get_context("spawn")
     t0 = time.perf_counter()
     with ctx.Pool(cores) as pool:
-        done = sum(pool.map(_oracle_job, [(min(C, 32 * n_groups), g, n_int) for g in range(n_groups)]))
+        done = sum(pool.map(_oracle_job, [(name, min(C, 32 * n_groups), g, n_int) for g in range(n_groups)]))
     wall = time.perf_counter() - t0
     return {"value": done / wall, "unit": "cell-intervals/s", "cores": cores, "kind": "port",
             "sample": f"oracle (NumPy/SciPy SuperLU restatement), first {n_groups} groups x {n_int} "
@@ -149,7 +161,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=48)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--cells", type=int, default=65536)
+    ap.add_argument("--cells", type=int, default=0, help="0: the workload's own size")
+    ap.add_argument("--workload", choices=("diurnal", "uneven", "constrained"), default="diurnal",
+                    help="diurnal: the headline 200-species workload; uneven: 262,144 cells of uneven "
+                         "stiffness; constrained: RODAS3 with conservation rows (100 species)")
     ap.add_argument("--plain-groups", type=int, default=8)
     ap.add_argument("--cpu-groups", type=int, default=0, help="0: 2 per core")
     ap.add_argument("--cpu-intervals", type=int, default=1)
@@ -172,7 +187,6 @@ def main():
     import __graft_entry__
     fa = __graft_entry__.load_package()
     from fastode_amd import workloads as W
-    from fastode_amd.mechanism import diurnal_factor
     from fastode_amd.system import ChemSystem
     from fastode_amd.device import mechanism_plan
 
@@ -187,14 +201,19 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes_read": xs.numel() * 8, "copy_probe_bytes_written": ys.numel() * 8}))
         del xs, ys
-    wl = W.diurnal200(C=args.cells, n_intervals=args.steps)
+    if args.workload == "diurnal":
+        wl = W.diurnal200(C=args.cells or 65536, n_intervals=args.steps)
+    elif args.workload == "uneven":
+        wl = W.uneven200(C=args.cells or 262144, n_intervals=args.steps)
+    else:
+        wl = W.constrained100(C=args.cells or 65536, n_intervals=args.steps)
+    tab = fa.ROS3 if wl.tableau == "ros3" else fa.RODAS3
     mech = wl.mech
     K = args.steps
-
-    def kmul(it):
-        m = np.ones(wl.R)
-        m[wl.photolysis] = diurnal_factor(it * wl.dt)
-        return m
+    # rate constants resident on the device: one shared row, or per cell
+    k_dev = wl.k_base[None, :] if wl.cell_scale is None else wl.k_base[None, :] * wl.cell_scale
+    k_dev = torch.as_tensor(k_dev, device=dev)
+    kmul = wl.kmul
 
     tuned = None
     if not args.no_tune:
@@ -202,12 +221,13 @@ def main():
         from fastode_amd.autotune import tune
         n_s = min(wl.C, 8192)
         t_tune = time.perf_counter()
-        ch = tune(mech, wl.k_base[None, :], wl.Y0[:n_s], wl.dt, wl.atol, wl.rtol, kmul=kmul(0), use_cache=False)
+        ch = tune(mech, k_dev[:n_s] if k_dev.shape[0] > 1 else k_dev, wl.Y0[:n_s], wl.dt, wl.atol, wl.rtol,
+                  tableau=tab, kmul=kmul(0), mass=wl.mass, use_cache=False)
         tuned = {"rho": ch.rho, "m_max": ch.m_max, "stage_tables": ch.stage_tables, "c": ch.c, "tile": ch.tile,
                  "us_per_cell_attempt": round(ch.us_per_cell_attempt, 3),
                  "tuning_s": round(time.perf_counter() - t_tune, 1)}
-    system = ChemSystem(mech, wl.k_base[None, :])
-    balg = fa.BatchedAlgorithm(fa.ROS3, blocks_per_cu=args.blocks_per_cu, longest_first=not args.no_order)
+    system = ChemSystem(mech, k_dev, mass=wl.mass)
+    balg = fa.BatchedAlgorithm(tab, blocks_per_cu=args.blocks_per_cu, longest_first=not args.no_order)
     plan = mechanism_plan(mech).plan
 
     # warmup on a copy of the batch
@@ -250,7 +270,7 @@ def main():
     # FETCH_SIZE + WRITE_SIZE of this kernel, scaled by a copy kernel's
     # reading) times this run's cell-attempts per launch
     traffic = None
-    pmc = ROOT / "profiles" / "traffic_mech200-diurnal.json"
+    pmc = ROOT / "profiles" / f"traffic_{wl.name}.json"
     if pmc.exists():
         t = json.loads(pmc.read_text())
         per_attempt = t.get("hbm_bytes_per_cell_attempt")
@@ -267,9 +287,9 @@ def main():
            "steps": K, "warmup": args.warmup, "ms_per_step": round(1e3 * wall / K, 3),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
            "data": "synthetic (seeded mechanism and states)",
-           "config": {"workload": "mech200-diurnal", "species": wl.n, "reactions": wl.R, "cells": wl.C,
+           "config": {"workload": wl.name, "species": wl.n, "reactions": wl.R, "cells": wl.C,
                       "group_size": 32, "groups": run.ngroups, "intervals": K, "dt_s": wl.dt,
-                      "rtol": wl.rtol, "atol": float(wl.atol[0]), "tableau": "ros3",
+                      "rtol": wl.rtol, "atol": float(wl.atol[0]), "tableau": wl.tableau,
                       "form": "batched persistent kernel, one hipGraph replay per interval",
                       "lu": {"nnz_J": mech.nnz, "nnz_LU": nnz_lu, "core": plan.m, "tile": plan.tile},
                       "autotuned": tuned},
@@ -279,7 +299,7 @@ def main():
            "roofline": roofline}
 
     if not args.no_plain:
-        alg = fa.Algorithm(fa.ROS3)
+        alg = fa.Algorithm(tab)
         G = min(args.plain_groups, run.ngroups)
         Yp = wl.Y0[: 32 * G].copy()
         mems = [alg.new_memory() for _ in range(G)]
@@ -287,7 +307,7 @@ def main():
         systems = [[None] * G for _ in range(K)]
         for it in range(K):
             for g in range(G):
-                systems[it][g] = ChemSystem(mech, wl.k(it, np.arange(32 * g, 32 * g + 32)))
+                systems[it][g] = ChemSystem(mech, wl.k(it, np.arange(32 * g, 32 * g + 32)), mass=wl.mass)
         Yd = torch.as_tensor(Yp, device=dev)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
@@ -313,7 +333,7 @@ def main():
     if not args.no_cpu:
         cores = max(1, min(16, len(os.sched_getaffinity(0))))
         ng = args.cpu_groups or 2 * cores
-        out["cpu_baseline"] = cpu_baseline(wl.C, ng, args.cpu_intervals)
+        out["cpu_baseline"] = cpu_baseline(args.workload, wl.C, ng, args.cpu_intervals)
     print(json.dumps(out))
 
 

@@ -93,7 +93,8 @@ def install(mech, choice: Choice):
     return mech._plan
 
 
-def tune(mech, k, Y, dt, atol, rtol, tableau=None, kmul=None, use_cache=True, verbose=False) -> Choice:
+def tune(mech, k, Y, dt, atol, rtol, tableau=None, kmul=None, mass=None, use_cache=True,
+         verbose=False) -> Choice:
     """Time the candidate plans on the sample (k (C, R) or (1, R), Y (C, n):
     the caller's cells, C about one group per CU or more) over one interval
     and install the fastest."""
@@ -114,7 +115,7 @@ def tune(mech, k, Y, dt, atol, rtol, tableau=None, kmul=None, use_cache=True, ve
         for stage in (True, False):
             balg = BatchedAlgorithm(tableau, stage_tables=stage)
             try:
-                system = ChemSystem(mech, k)
+                system = ChemSystem(mech, k, mass=mass)
                 run = balg.runner(system, Y, dt, atol, rtol, graph=False)
                 ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
                 torch.cuda.synchronize()
