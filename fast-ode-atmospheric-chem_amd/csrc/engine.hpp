@@ -239,15 +239,25 @@ __device__ inline void schur(const PlanDev& P, const Tables& T, const double* ld
   const uint32_t mw[4] = {mk.x, mk.y, mk.z, mk.w};
   j = 0;
 #pragma unroll
-  for (int ti = 0; ti < TILE; ++ti)
+  for (int ti = 0; ti < TILE; ++ti) {
+    uint32_t row = 0;
 #pragma unroll
     for (int tj = 0; tj < TILE; ++tj) {
       const int s = ti * TILE + tj;
-      const bool has = (mw[s >> 5] >> (s & 31)) & 1u;
-      const double sk = S[j * NT + tid];   // read always (S has sch_depth + 1 rows)
-      a[ti][tj] = has ? a[ti][tj] + sk : a[ti][tj];
-      j += has ? 1 : 0;
+      row |= ((mw[s >> 5] >> (s & 31)) & 1u) << tj;
     }
+    // one branch per tile row: a row's loads issue together (S has
+    // sch_depth + 1 rows, so the unmasked reads stay inside it)
+    if (row) {
+#pragma unroll
+      for (int tj = 0; tj < TILE; ++tj) {
+        const uint32_t bit = (row >> tj) & 1u;
+        const long long raw = __double_as_longlong(S[j * NT + tid]);
+        a[ti][tj] = a[ti][tj] + __longlong_as_double(raw & -(long long)bit);
+        j += bit;
+      }
+    }
+  }
 }
 
 // Publish pivot k1 = NB*G + kk1 (raw column, row, and the pivot's
