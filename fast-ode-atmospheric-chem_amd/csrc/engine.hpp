@@ -8,7 +8,7 @@
 //     registers, 2-D cyclic over a 16 x 16 thread grid: thread (tr, tc) owns
 //     core entries (ti*16 + tr, tj*16 + tc), ti, tj < TILE.
 // The core is factored right-looking without pivoting with one barrier per
-// pivot (double-buffered pivot row/column in LDS); its 16 x 16 diagonal
+// two pivots (double-buffered pivot rows/columns in LDS); its 16 x 16 diagonal
 // blocks are then inverted into LDS so that each triangular solve is TILE
 // block steps of one barrier each.
 #pragma once
@@ -21,8 +21,8 @@ namespace fode {
 // LDS map of the cell engine (doubles), sized by the host from the plan.
 struct EngineLds {
   double* lds;    // [lds_len]: sparse entries, then the solve vector (c + mpad)
-  double* lbuf;   // [2*mpad]   pivot column, double-buffered
-  double* ubuf;   // [2*mpad]   pivot row
+  double* lbuf;   // [4*mpad]   pivot columns, two per pair, double-buffered
+  double* ubuf;   // [4*mpad]   pivot rows
   double* rbuf;   // [mpad]     pivot reciprocals
   double* pb;     // [2*NT]     block-solve partial products
   double* scr;    // [tile*NT]  diagonal blocks being inverted
@@ -34,7 +34,7 @@ struct EngineLds {
 // (while the Schur update runs, before any of these is live; the pivot
 // buffers are written as soon as a thread leaves it) the Schur sums,
 // thread-interleaved S[j * NT + t] from pb on.
-__host__ __device__ inline int engine_pb_offset(const PlanDev& P) { return P.lds_len + 5 * P.mpad; }
+__host__ __device__ inline int engine_pb_offset(const PlanDev& P) { return P.lds_len + 9 * P.mpad; }
 __host__ __device__ inline int engine_scr_offset(const PlanDev& P) { return engine_pb_offset(P) + 2 * NT; }
 
 __host__ __device__ inline int engine_lds_doubles(const PlanDev& P) {
@@ -47,8 +47,8 @@ __device__ inline EngineLds carve(double* base, const PlanDev& P) {
   EngineLds e;
   e.lds = base;
   e.lbuf = e.lds + P.lds_len;
-  e.ubuf = e.lbuf + 2 * P.mpad;
-  e.rbuf = e.ubuf + 2 * P.mpad;
+  e.ubuf = e.lbuf + 4 * P.mpad;
+  e.rbuf = e.ubuf + 4 * P.mpad;
   e.pb = e.rbuf + P.mpad;
   e.scr = e.pb + 2 * NT;
   e.dL = e.scr + P.tile * NT;
@@ -260,25 +260,32 @@ __device__ inline void schur(const PlanDev& P, const Tables& T, const double* ld
   }
 }
 
-// Publish pivot k1 = NB*G + kk1 (raw column, row, and the pivot's
-// reciprocal at rcs[k1]) into buffer nb.
+// Publish the current column and row of pivots k0 = NB*G + kk and k0 + 1
+// (updated through pivot k0 - 1) into pair buffer nb, and the reciprocal of
+// pivot k0 (final at that point) at rcs[k0].
 template <int TILE, int NB>
-__device__ inline void publish(const double (&a)[TILE][TILE], int kk1, int nb, double* lbuf,
-                               double* ubuf, double* rcs, int tr, int tc) {
+__device__ inline void publish_pair(const double (&a)[TILE][TILE], int kk, int nb, double* lbuf,
+                                    double* ubuf, double* rcs, int tr, int tc) {
   constexpr int MP = G * TILE;
-  if (tc == kk1) {
+  double* c = lbuf + 2 * nb * MP;
+  double* r = ubuf + 2 * nb * MP;
+  const int dc = tc - kk, dr = tr - kk;
+  if (dc == 0 || dc == 1) {
 #pragma unroll
-    for (int ti = NB; ti < TILE; ++ti) lbuf[nb * MP + ti * G + tr] = a[ti][NB];
+    for (int ti = NB; ti < TILE; ++ti) c[dc * MP + ti * G + tr] = a[ti][NB];
   }
-  if (tr == kk1) {
+  if (dr == 0 || dr == 1) {
 #pragma unroll
-    for (int tj = NB; tj < TILE; ++tj) ubuf[nb * MP + tj * G + tc] = a[NB][tj];
-    if (tc == kk1) rcs[NB * G + kk1] = 1.0 / a[NB][NB];
+    for (int tj = NB; tj < TILE; ++tj) r[dr * MP + tj * G + tc] = a[NB][tj];
+    if (dr == 0 && dc == 0) rcs[NB * G + kk] = 1.0 / a[NB][NB];
   }
 }
 
-// The 16 pivots of tile block B.  Only tile row/column B straddle the
-// pivot, so only there are the multipliers and pivot-row values masked;
+// The 16 pivots of tile block B, two per barrier.  Every thread applies
+// pivot k0 to the published column and row of k1 = k0 + 1 itself (the same
+// fma the owners would do, so the bits are those of one pivot per barrier),
+// then both rank-1 updates in order.  Only tile row/column B straddle the
+// pivots, so only there are the multipliers and pivot-row values masked;
 // the owners of the pivot columns keep their raw entries (the update leaves
 // them alone: their u is 0) and scale them by the pivots' reciprocals once,
 // after the block -- the same product l = raw * (1/pivot) the consumers use.
@@ -287,27 +294,43 @@ __device__ inline void factor_block(double (&a)[TILE][TILE], double* lbuf, doubl
                                     double* rcs, int tr, int tc) {
   constexpr int MP = G * TILE;
 #pragma nounroll
-  for (int kk = 0; kk < G; ++kk) {
-    const int k = B * G + kk;
-    const int buf = k & 1;
-    const double rc = rcs[k];
-    const double* lb = lbuf + buf * MP + tr;
-    const double* ub = ubuf + buf * MP + tc;
-    double u[TILE];
-    u[B] = (tc > kk) ? ub[B * G] : 0.0;
+  for (int kk = 0; kk < G; kk += 2) {
+    const int k0 = B * G + kk;
+    const int buf = (k0 >> 1) & 1;
+    const double* c0 = lbuf + 2 * buf * MP;
+    const double* c1 = c0 + MP;
+    const double* r0 = ubuf + 2 * buf * MP;
+    const double* r1 = r0 + MP;
+    const double rc0 = rcs[k0];
+    const double l10 = c0[B * G + kk + 1] * rc0;   // multiplier of row k1 for pivot k0
+    const double u01 = r0[B * G + kk + 1];         // row k0 at column k1
+    const double rc1 = 1.0 / fma(-l10, u01, r1[B * G + kk + 1]);
+    if (tr == kk + 1 && tc == kk + 1) rcs[k0 + 1] = rc1;
+    double u0[TILE], u1[TILE];
 #pragma unroll
-    for (int tj = B + 1; tj < TILE; ++tj) u[tj] = ub[tj * G];
+    for (int tj = B; tj < TILE; ++tj) {
+      const double x0 = r0[tj * G + tc];
+      u0[tj] = x0;
+      u1[tj] = fma(-l10, x0, r1[tj * G + tc]);
+    }
+    u0[B] = (tc > kk) ? u0[B] : 0.0;
+    u1[B] = (tc > kk + 1) ? u1[B] : 0.0;
 #pragma unroll
     for (int ti = B; ti < TILE; ++ti) {
-      double l = lb[ti * G] * rc;
-      if (ti == B) l = (tr > kk) ? l : 0.0;
+      double l0 = c0[ti * G + tr] * rc0;
+      if (ti == B) l0 = (tr > kk) ? l0 : 0.0;
+      double l1 = fma(-l0, u01, c1[ti * G + tr]) * rc1;
+      if (ti == B) l1 = (tr > kk + 1) ? l1 : 0.0;
 #pragma unroll
-      for (int tj = B; tj < TILE; ++tj) a[ti][tj] = fma(-l, u[tj], a[ti][tj]);
+      for (int tj = B; tj < TILE; ++tj) {
+        a[ti][tj] = fma(-l0, u0[tj], a[ti][tj]);
+        a[ti][tj] = fma(-l1, u1[tj], a[ti][tj]);
+      }
     }
-    if (kk + 1 < G) {
-      publish<TILE, B>(a, kk + 1, buf ^ 1, lbuf, ubuf, rcs, tr, tc);
+    if (kk + 2 < G) {
+      publish_pair<TILE, B>(a, kk + 2, buf ^ 1, lbuf, ubuf, rcs, tr, tc);
     } else if constexpr (B + 1 < TILE) {
-      publish<TILE, B + 1>(a, 0, buf ^ 1, lbuf, ubuf, rcs, tr, tc);
+      publish_pair<TILE, B + 1>(a, 0, buf ^ 1, lbuf, ubuf, rcs, tr, tc);
     }
     __syncthreads();
   }
@@ -337,7 +360,7 @@ __device__ inline void factor_blocks(double (&a)[TILE][TILE], double* lbuf, doub
 template <int TILE>
 __device__ inline void core_factor(double (&a)[TILE][TILE], const EngineLds& L) {
   const int tid = threadIdx.x, tr = tid / G, tc = tid % G;
-  publish<TILE, 0>(a, 0, 0, L.lbuf, L.ubuf, L.rbuf, tr, tc);
+  publish_pair<TILE, 0>(a, 0, 0, L.lbuf, L.ubuf, L.rbuf, tr, tc);
   __syncthreads();
   factor_blocks<TILE, 0>(a, L.lbuf, L.ubuf, L.rbuf, tr, tc);
 }
