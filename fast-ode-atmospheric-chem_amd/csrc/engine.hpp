@@ -89,7 +89,21 @@ __device__ inline void run_tasks(const uint2* tasks, const int* ptr, int nst, co
       const uint32_t tgt = tk.x & 0xFFFFu, div = tk.x >> 16;
       const uint32_t q0 = tk.y & 0xFFFFFu, cnt = tk.y >> 20;
       double acc = lds[tgt];
-      for (uint32_t q = q0; q < q0 + cnt; ++q) {
+      const uint32_t qe = q0 + cnt;
+      uint32_t q = q0;
+      // four pairs' loads in flight at a time; the fmas keep the pair order
+      for (; q + 4 <= qe; q += 4) {
+        const uint32_t w0 = pairs[q], w1 = pairs[q + 1], w2 = pairs[q + 2], w3 = pairs[q + 3];
+        const double x0 = lds[w0 >> 16], y0 = lds[w0 & 0xFFFFu];
+        const double x1 = lds[w1 >> 16], y1 = lds[w1 & 0xFFFFu];
+        const double x2 = lds[w2 >> 16], y2 = lds[w2 & 0xFFFFu];
+        const double x3 = lds[w3 >> 16], y3 = lds[w3 & 0xFFFFu];
+        acc = fma(-x0, y0, acc);
+        acc = fma(-x1, y1, acc);
+        acc = fma(-x2, y2, acc);
+        acc = fma(-x3, y3, acc);
+      }
+      for (; q < qe; ++q) {
         const uint32_t w = pairs[q];
         acc = fma(-lds[w >> 16], lds[w & 0xFFFFu], acc);
       }
