@@ -80,35 +80,86 @@ __device__ inline double rowsum16(double x) {
 // Level-scheduled dot-product tasks over LDS:
 //   lds[t] = (lds[t] - sum_q lds[a_q] * lds[b_q]) [/ lds[d]]
 // Tasks of one stage are independent; a barrier follows every stage.
+
+// One task per thread: the pairs in order, four pairs' loads in flight at a
+// time.
+__device__ inline void task_single(const uint2 tk, const uint32_t* pairs, double* lds) {
+  const uint32_t tgt = tk.x & 0xFFFFu, div = tk.x >> 16;
+  const uint32_t q0 = tk.y & 0xFFFFFu, qe = q0 + (tk.y >> 20);
+  double acc = lds[tgt];
+  uint32_t q = q0;
+  for (; q + 4 <= qe; q += 4) {
+    const uint32_t w0 = pairs[q], w1 = pairs[q + 1], w2 = pairs[q + 2], w3 = pairs[q + 3];
+    const double x0 = lds[w0 >> 16], y0 = lds[w0 & 0xFFFFu];
+    const double x1 = lds[w1 >> 16], y1 = lds[w1 & 0xFFFFu];
+    const double x2 = lds[w2 >> 16], y2 = lds[w2 & 0xFFFFu];
+    const double x3 = lds[w3 >> 16], y3 = lds[w3 & 0xFFFFu];
+    acc = fma(-x0, y0, acc);
+    acc = fma(-x1, y1, acc);
+    acc = fma(-x2, y2, acc);
+    acc = fma(-x3, y3, acc);
+  }
+  for (; q < qe; ++q) {
+    const uint32_t w = pairs[q];
+    acc = fma(-lds[w >> 16], lds[w & 0xFFFFu], acc);
+  }
+  if (div != NO_DIV) acc = acc / lds[div];
+  lds[tgt] = acc;
+}
+
+// One task per quad (stages of at most NT/4 tasks, which are the long ones
+// in the solves): lane j of the quad sums the pairs q0 + j, q0 + j + 4, ...
+// in order, the four partial sums are combined by DPP quad permutes (the
+// same bits in all four lanes), and lane 0 writes
+//   lds[t] = (lds[t] - ((s0 + s1) + (s2 + s3))) [/ lds[d]].
+// Called with every lane of the workgroup active (the DPP reads them all).
+__device__ inline void task_quad(const uint2* tasks, int t, int e, const uint32_t* pairs, double* lds) {
+  const int j = threadIdx.x & 3;
+  const bool live = t < e;
+  double part = 0.0;
+  uint32_t tgt = 0, div = NO_DIV;
+  if (live) {
+    const uint2 tk = tasks[t];
+    tgt = tk.x & 0xFFFFu;
+    div = tk.x >> 16;
+    const uint32_t q0 = tk.y & 0xFFFFFu, qe = q0 + (tk.y >> 20);
+    uint32_t q = q0 + j;
+    for (; q + 12 < qe; q += 16) {
+      const uint32_t w0 = pairs[q], w1 = pairs[q + 4], w2 = pairs[q + 8], w3 = pairs[q + 12];
+      const double x0 = lds[w0 >> 16], y0 = lds[w0 & 0xFFFFu];
+      const double x1 = lds[w1 >> 16], y1 = lds[w1 & 0xFFFFu];
+      const double x2 = lds[w2 >> 16], y2 = lds[w2 & 0xFFFFu];
+      const double x3 = lds[w3 >> 16], y3 = lds[w3 & 0xFFFFu];
+      part = fma(x0, y0, part);
+      part = fma(x1, y1, part);
+      part = fma(x2, y2, part);
+      part = fma(x3, y3, part);
+    }
+    for (; q < qe; q += 4) {
+      const uint32_t w = pairs[q];
+      part = fma(lds[w >> 16], lds[w & 0xFFFFu], part);
+    }
+  }
+  part += dpp_mov<0xB1>(part);   // quad_perm [1, 0, 3, 2]
+  part += dpp_mov<0x4E>(part);   // quad_perm [2, 3, 0, 1]
+  if (live && j == 0) {
+    double acc = lds[tgt] - part;
+    if (div != NO_DIV) acc = acc / lds[div];
+    lds[tgt] = acc;
+  }
+}
+
+// quad: stages of at most NT/4 tasks take a quad per task (uniform choice
+// per stage, made from the stage's size alone).
+template <bool QUAD>
 __device__ inline void run_tasks(const uint2* tasks, const int* ptr, int nst, const uint32_t* pairs,
                                  double* lds) {
   for (int s = 0; s < nst; ++s) {
     const int b = ptr[s], e = ptr[s + 1];
-    for (int t = b + (int)threadIdx.x; t < e; t += NT) {
-      const uint2 tk = tasks[t];
-      const uint32_t tgt = tk.x & 0xFFFFu, div = tk.x >> 16;
-      const uint32_t q0 = tk.y & 0xFFFFFu, cnt = tk.y >> 20;
-      double acc = lds[tgt];
-      const uint32_t qe = q0 + cnt;
-      uint32_t q = q0;
-      // four pairs' loads in flight at a time; the fmas keep the pair order
-      for (; q + 4 <= qe; q += 4) {
-        const uint32_t w0 = pairs[q], w1 = pairs[q + 1], w2 = pairs[q + 2], w3 = pairs[q + 3];
-        const double x0 = lds[w0 >> 16], y0 = lds[w0 & 0xFFFFu];
-        const double x1 = lds[w1 >> 16], y1 = lds[w1 & 0xFFFFu];
-        const double x2 = lds[w2 >> 16], y2 = lds[w2 & 0xFFFFu];
-        const double x3 = lds[w3 >> 16], y3 = lds[w3 & 0xFFFFu];
-        acc = fma(-x0, y0, acc);
-        acc = fma(-x1, y1, acc);
-        acc = fma(-x2, y2, acc);
-        acc = fma(-x3, y3, acc);
-      }
-      for (; q < qe; ++q) {
-        const uint32_t w = pairs[q];
-        acc = fma(-lds[w >> 16], lds[w & 0xFFFFu], acc);
-      }
-      if (div != NO_DIV) acc = acc / lds[div];
-      lds[tgt] = acc;
+    if (QUAD && 4 * (e - b) <= NT) {
+      task_quad(tasks, b + (int)(threadIdx.x >> 2), e, pairs, lds);
+    } else {
+      for (int t = b + (int)threadIdx.x; t < e; t += NT) task_single(tasks[t], pairs, lds);
     }
     __syncthreads();
   }
@@ -419,16 +470,18 @@ __device__ inline void diag_inverses(const double (&a)[TILE][TILE], const Engine
   }
 }
 
-// Sum of p[q*G] for q < 16 as a tree (4 dependent adds, not 15).
-__device__ inline double sum16(const double* p) {
-  double t[8];
-#pragma unroll
-  for (int q = 0; q < 8; ++q) t[q] = p[q * G] + p[(q + 8) * G];
-#pragma unroll
-  for (int q = 0; q < 4; ++q) t[q] = t[q] + t[q + 4];
-  t[0] = t[0] + t[2];
-  t[1] = t[1] + t[3];
-  return t[0] + t[1];
+// Sum over the 16 grid rows of x (the threads of column tc): first over
+// the wave's four rows (lanes 16 and 32 apart) by lane shuffles, then over
+// the four waves' partials through p (one barrier; 4 LDS reads a thread, not
+// 16).  The same bits in every thread of the column.
+__device__ inline double colsum16(double x, double* p) {
+  const int tid = threadIdx.x, lane = tid & 63;
+  x += __shfl_xor(x, 16, 64);
+  x += __shfl_xor(x, 32, 64);
+  if (lane < G) p[(tid >> 6) * G + lane] = x;
+  __syncthreads();
+  const int tc = tid % G;
+  return (p[tc] + p[G + tc]) + (p[2 * G + tc] + p[3 * G + tc]);
 }
 
 // Block forward/back substitution of the core: v (length mpad) in LDS holds
@@ -444,10 +497,7 @@ __device__ inline void core_solve(const double (&a)[TILE][TILE], const EngineLds
     const double d = L.dL[J * NT + tid];
     const double s = rowsum16(acc[J]);
     const double r = v[J * G + tr] - s;
-    double* p = L.pb + (J & 1) * NT;
-    p[tid] = d * r;
-    __syncthreads();
-    const double y = sum16(p + tc);
+    const double y = colsum16(d * r, L.pb + (J & 1) * NT);
     if (tr == 0) v[J * G + tc] = y;
 #pragma unroll
     for (int I = J + 1; I < TILE; ++I) acc[I] = fma(a[I][J], y, acc[I]);
@@ -460,10 +510,7 @@ __device__ inline void core_solve(const double (&a)[TILE][TILE], const EngineLds
     const double d = L.dU[J * NT + tid];
     const double s = rowsum16(acc[J]);
     const double r = v[J * G + tr] - s;
-    double* p = L.pb + (J & 1) * NT;
-    p[tid] = d * r;
-    __syncthreads();
-    const double x = sum16(p + tc);
+    const double x = colsum16(d * r, L.pb + (J & 1) * NT);
     if (tr == 0) v[J * G + tc] = x;
 #pragma unroll
     for (int I = 0; I < J; ++I) acc[I] = fma(a[I][J], x, acc[I]);
@@ -476,7 +523,7 @@ template <int TILE>
 __device__ __forceinline__ void factor_cell(const PlanDev& P, const Tables& T, const EngineLds& L,
                                    double (&a)[TILE][TILE]) {
   __syncthreads();
-  run_tasks(T.f_tasks, T.f_ptr, P.f_nst, T.f_pairs, L.lds);
+  run_tasks<false>(T.f_tasks, T.f_ptr, P.f_nst, T.f_pairs, L.lds);
   schur<TILE>(P, T, L.lds, L.pb, a);
   core_factor<TILE>(a, L);
   diag_inverses<TILE>(a, L);
@@ -489,9 +536,9 @@ __device__ __forceinline__ void solve_cell(const PlanDev& P, const Tables& T, co
                                   const double (&a)[TILE][TILE]) {
   double* vec = L.lds + P.vec;
   __syncthreads();
-  run_tasks(T.sf_tasks, T.sf_ptr, P.sf_nst, T.s_pairs, L.lds);
+  run_tasks<true>(T.sf_tasks, T.sf_ptr, P.sf_nst, T.s_pairs, L.lds);
   core_solve<TILE>(a, L, vec + P.c);
-  run_tasks(T.sb_tasks, T.sb_ptr, P.sb_nst, T.s_pairs, L.lds);
+  run_tasks<true>(T.sb_tasks, T.sb_ptr, P.sb_nst, T.s_pairs, L.lds);
 }
 
 }  // namespace fode
