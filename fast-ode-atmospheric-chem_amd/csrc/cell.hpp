@@ -86,22 +86,104 @@ struct JLds {
   __device__ double operator()(int e) const { return J[e]; }
 };
 
+// The cell's rates, forcing and Jacobian, with the same arithmetic (and
+// bits) as rate/forcing/drate/JFromD, but several items per thread per
+// round so that their table loads (global) and LDS gathers are in flight
+// together instead of one dependent chain after another.  Two-reactant-slot
+// mechanisms (P == 2) take straight-line paths; others the generic ones.
+
 __device__ inline void rates_into(const MechDev& M, const double* y, const double* kc, double* rates) {
-  for (int j = threadIdx.x; j < M.R; j += NT) rates[j] = rate(M, y, kc, j);
+  if (M.P == 2) {
+    for (int j0 = threadIdx.x; j0 < M.R; j0 += 2 * NT) {
+      double r[2];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int j = j0 + u * NT, jj = j < M.R ? j : 0;
+        const int s0 = M.react[2 * jj], s1 = M.react[2 * jj + 1];
+        const double v0 = s0 < M.n ? y[s0] : 1.0, v1 = s1 < M.n ? y[s1] : 1.0;
+        r[u] = kc[jj] * (v0 * v1);
+      }
+#pragma unroll
+      for (int u = 0; u < 2; ++u)
+        if (j0 + u * NT < M.R) rates[j0 + u * NT] = r[u];
+    }
+  } else {
+    for (int j = threadIdx.x; j < M.R; j += NT) rates[j] = rate(M, y, kc, j);
+  }
   __syncthreads();
 }
 
 __device__ inline void forcing_into(const MechDev& M, const double* y, const double* rates, double* out) {
-  for (int i = threadIdx.x; i < M.n; i += NT) out[i] = forcing(M, y, rates, i);
+  for (int i = threadIdx.x; i < M.n; i += NT) {
+    if (M.con_of_row[i] >= 0) {
+      out[i] = forcing(M, y, rates, i);
+      continue;
+    }
+    double f = 0.0;
+    int q = M.rhs_ptr[i];
+    const int qe = M.rhs_ptr[i + 1];
+    for (; q + 4 <= qe; q += 4) {
+      const int x0 = M.rhs_rxn[q], x1 = M.rhs_rxn[q + 1], x2 = M.rhs_rxn[q + 2], x3 = M.rhs_rxn[q + 3];
+      const double c0 = M.rhs_coef[q], c1 = M.rhs_coef[q + 1], c2 = M.rhs_coef[q + 2], c3 = M.rhs_coef[q + 3];
+      const double r0 = rates[x0], r1 = rates[x1], r2 = rates[x2], r3 = rates[x3];
+      f = f + r0 * c0;
+      f = f + r1 * c1;
+      f = f + r2 * c2;
+      f = f + r3 * c3;
+    }
+    for (; q < qe; ++q) f = f + rates[M.rhs_rxn[q]] * M.rhs_coef[q];
+    out[i] = f;
+  }
   __syncthreads();
 }
 
 // Rate partials at V.y into V.rd, then the Jacobian values into V.jv.
 __device__ inline void jacobian_into(const MechDev& M, const CellLds& V) {
-  for (int jp = threadIdx.x; jp < M.R * M.P; jp += NT) V.rd[jp] = drate(M, V.y, V.kc, jp / M.P, jp % M.P);
+  const int tid = threadIdx.x, RP = M.R * M.P;
+  if (M.P == 2) {
+    // d rate_j / d y_{react[j,p]} = k_j * y_{react[j,1-p]}
+    for (int jp0 = tid; jp0 < RP; jp0 += 4 * NT) {
+      double d[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int jp = jp0 + u * NT, jj = jp < RP ? jp : 0;
+        const int s = M.react[jj ^ 1];
+        d[u] = V.kc[jj >> 1] * (s < M.n ? V.y[s] : 1.0);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (jp0 + u * NT < RP) V.rd[jp0 + u * NT] = d[u];
+    }
+  } else {
+    for (int jp = tid; jp < RP; jp += NT) V.rd[jp] = drate(M, V.y, V.kc, jp / M.P, jp % M.P);
+  }
   __syncthreads();
-  const JFromD jd{M, V.rd};
-  for (int e = threadIdx.x; e < M.nnz; e += NT) V.jv[e] = jd(e);
+  // entries: the first term of four entries at once (most entries have one;
+  // the term arrays hold at least one element, so index 0 stands in for an
+  // entry without terms), then any further terms in order
+  for (int e0 = tid; e0 < M.nnz; e0 += 4 * NT) {
+    int qa[4], qb[4];
+    double v[4], c[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = e0 + u * NT, ee = e < M.nnz ? e : 0;
+      qa[u] = M.jac_ptr[ee];
+      qb[u] = M.jac_ptr[ee + 1];
+      c[u] = M.jac_const[ee];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const bool has = qa[u] < qb[u];
+      const int q = has ? qa[u] : 0;
+      const double t = M.jac_coef[q] * V.rd[M.jac_src[q]];
+      v[u] = has ? 0.0 + t : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      for (int q = qa[u] + 1; q < qb[u]; ++q) v[u] = v[u] + M.jac_coef[q] * V.rd[M.jac_src[q]];
+      if (e0 + u * NT < M.nnz) V.jv[e0 + u * NT] = v[u] + c[u];
+    }
+  }
   __syncthreads();
 }
 

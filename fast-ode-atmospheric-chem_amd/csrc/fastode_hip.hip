@@ -39,6 +39,15 @@ int upload(const T* host, size_t count, T** dev, std::vector<void*>& owned) {
   return 0;
 }
 
+// upload, but never empty: a zero element stands in for an empty array, so
+// that kernels may read index 0 of a term table unconditionally.
+template <class T>
+int upload_nonempty(const T* host, size_t count, T** dev, std::vector<void*>& owned) {
+  if (count > 0) return upload(host, count, dev, owned);
+  const T zero{};
+  return upload(&zero, 1, dev, owned);
+}
+
 struct MechHandle {
   MechDev d;
   std::vector<void*> owned;
@@ -321,11 +330,11 @@ int fode_mech_create(const fode_mech_desc* d, void** out) {
   int rc = 0;
   rc |= upload(d->react, (size_t)d->R * d->P, (int**)&m.react, h->owned);
   rc |= upload(d->rhs_ptr, (size_t)d->n + 1, (int**)&m.rhs_ptr, h->owned);
-  rc |= upload(d->rhs_rxn, (size_t)d->rhs_ptr[d->n], (int**)&m.rhs_rxn, h->owned);
-  rc |= upload(d->rhs_coef, (size_t)d->rhs_ptr[d->n], (double**)&m.rhs_coef, h->owned);
+  rc |= upload_nonempty(d->rhs_rxn, (size_t)d->rhs_ptr[d->n], (int**)&m.rhs_rxn, h->owned);
+  rc |= upload_nonempty(d->rhs_coef, (size_t)d->rhs_ptr[d->n], (double**)&m.rhs_coef, h->owned);
   rc |= upload(d->jac_ptr, (size_t)d->nnz + 1, (int**)&m.jac_ptr, h->owned);
-  rc |= upload(d->jac_src, (size_t)d->jac_ptr[d->nnz], (int**)&m.jac_src, h->owned);
-  rc |= upload(d->jac_coef, (size_t)d->jac_ptr[d->nnz], (double**)&m.jac_coef, h->owned);
+  rc |= upload_nonempty(d->jac_src, (size_t)d->jac_ptr[d->nnz], (int**)&m.jac_src, h->owned);
+  rc |= upload_nonempty(d->jac_coef, (size_t)d->jac_ptr[d->nnz], (double**)&m.jac_coef, h->owned);
   rc |= upload(d->jac_const, (size_t)d->nnz, (double**)&m.jac_const, h->owned);
   std::vector<int> con_of(d->n, -1);
   for (int c = 0; c < d->ncon; ++c) {
