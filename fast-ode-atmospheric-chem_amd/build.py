@@ -5,15 +5,16 @@ from __future__ import annotations
 import os
 import subprocess
 import sys
+import tempfile
 from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
-SRCS = [HERE / "csrc" / "fastode_hip.hip", HERE / "csrc" / "batched.hip", HERE / "csrc" / "adjoint.hip"]
+SRCS = [HERE / "csrc" / f for f in ("fastode_hip.hip", "batched.hip", "batched_unstaged.hip", "adjoint.hip")]
 LIB = HERE / "libfastode_hip.so"
 ARCH = os.environ.get("FODE_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = [
-    "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-O3", "-std=c++17", "-fPIC",
     f"--offload-arch={ARCH}",
     # The controller and stage arithmetic reproduce NumPy's separate
     # multiply and add roundings; fused multiply-adds are written as fma().
@@ -24,12 +25,26 @@ FLAGS = [
 
 
 def build(verbose=False) -> Path:
+    """Each source to an object in parallel (the batched kernels dominate),
+    then one shared library."""
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc, *FLAGS, "-I", str(HERE.parent / "include"), *map(str, SRCS), "-o", str(tmp)]
-    if verbose:
-        print(" ".join(cmd), file=sys.stderr)
-    subprocess.run(cmd, check=True)
+    with tempfile.TemporaryDirectory(prefix="fode_build_") as tmpdir:
+        objs, procs = [], []
+        for src in SRCS:
+            obj = Path(tmpdir) / (src.stem + ".o")
+            cmd = [hipcc, *FLAGS, "-I", str(HERE.parent / "include"), "-c", str(src), "-o", str(obj)]
+            if verbose:
+                print(" ".join(cmd), file=sys.stderr)
+            objs.append(obj)
+            procs.append((cmd, subprocess.Popen(cmd)))
+        failed = [cmd for cmd, pr in procs if pr.wait() != 0]
+        if failed:
+            raise subprocess.CalledProcessError(1, failed[0])
+        tmp = LIB.with_suffix(".so.tmp")
+        cmd = [hipcc, "-shared", "-fPIC", f"--offload-arch={ARCH}", *map(str, objs), "-o", str(tmp)]
+        if verbose:
+            print(" ".join(cmd), file=sys.stderr)
+        subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
     return LIB
 

@@ -11,375 +11,14 @@
 // cell's state, rate constants and new state touch memory.  The group's error
 // is the sum of the cells' partial sums in cell order, exactly as the plain
 // form forms it, so both forms take the same steps.
-#include <hip/hip_runtime.h>
-
-#include <string>
-
-#include "cell.hpp"
+#include "batched_kernel.hpp"
 
 using namespace fode;
 
-// Shared with fastode_hip.hip, which owns the handles and the error text.
-extern "C" const void* fode_internal_plan(void* plan);
-extern "C" const void* fode_internal_mech(void* mech);
-extern "C" void fode_internal_set_error(const char* msg);
-
 namespace {
-
-int bfail(const std::string& m) {
-  fode_internal_set_error(m.c_str());
-  return -1;
-}
-
-struct BatchArgs {
-  PlanDev P;
-  MechDev M;
-  fode_ctrl_params p;
-  fode_batch_desc b;
-  Layout lay;
-};
-
-// Tables in LDS when everything fits in one workgroup's LDS (and the caller
-// allows it), else in memory.
-Layout choose_layout(const PlanDev& P, const MechDev& M, int S, int options = 0) {
-  int dev = 0, maxlds = 0;
-  (void)hipGetDevice(&dev);
-  (void)hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
-  const Layout staged = make_layout(P, M, S, true);
-  if (!(options & 1) && (size_t)staged.total * sizeof(double) + sizeof(fode_group_ctrl) <= (size_t)maxlds)
-    return staged;
-  return make_layout(P, M, S, false);
-}
-
-// One attempt on one cell. Returns the cell's error partial sum (thread 0).
-template <int TILE>
-__device__ __forceinline__ double cell_attempt(const BatchArgs& A, const Tables& T, const EngineLds& L,
-                               const CellLds& V, const fode_group_ctrl& g, int cell, double* ynew_out) {
-  const MechDev& M = A.M;
-  const int n = M.n, tid = threadIdx.x;
-  const double* yg = A.b.Y + (size_t)cell * n;
-  const double* kg = A.b.k + (size_t)cell * A.b.k_stride;
-  for (int i = tid; i < n; i += NT) V.y[i] = yg[i];
-  for (int j = tid; j < M.R; j += NT) V.kc[j] = kg[j] * A.b.kmul[j];
-  __syncthreads();
-  cell_stages<TILE>(A.P, T, M, A.p, A.b.mass, L, V, g.H, g.alpha, ynew_out);
-  return cell_err_sum(V.y, V.w, V.fr, A.b.atol, A.b.rtol, n, V.red);
-}
-
-// Maximum over the workgroup of one double per thread, to every thread.
-__device__ inline double block_max_all(double x, double* red) {
-  for (int off = 32; off > 0; off >>= 1) x = fmax(x, __shfl_xor(x, off, 64));
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = x;
-  __syncthreads();
-  double m = red[0];
-  for (int i = 1; i < NT / 64; ++i) m = fmax(m, red[i]);
-  __syncthreads();
-  return m;
-}
-
-// Cell c of the group (state from Y, rate constants) into LDS.
-__device__ __forceinline__ void load_cell(const BatchArgs& A, const CellLds& V, int cell) {
-  const int n = A.M.n;
-  const double* kg = A.b.k + (size_t)cell * A.b.k_stride;
-  for (int i = threadIdx.x; i < n; i += NT) V.y[i] = A.b.Y[(size_t)cell * n + i];
-  for (int j = threadIdx.x; j < A.M.R; j += NT) V.kc[j] = kg[j] * A.b.kmul[j];
-  __syncthreads();
-}
-
-// The projection's residual at y (LDS): F_i(y) on algebraic rows, 0 on the
-// others (fastode/solvers.py:110-114), into g.
-__device__ __forceinline__ void residual_into(const BatchArgs& A, const CellLds& V, const double* y, double* g) {
-  rates_into(A.M, y, V.kc, V.rd);
-  for (int i = threadIdx.x; i < A.M.n; i += NT) g[i] = A.b.mass[i] == 0.0 ? forcing(A.M, y, V.rd, i) : 0.0;
-  __syncthreads();
-}
-
-// out = (M - J_alg(V.y))^{-1} rhs: the factorization with algebraic rows
-// only (fastode/linalg.py:60-67, alpha = 1) and one solve.
-template <int TILE>
-__device__ __forceinline__ void alg_solve(const BatchArgs& A, const Tables& T, const EngineLds& L, const CellLds& V,
-                          const double* rhs, double* out) {
-  const PlanDev& P = A.P;
-  const int n = A.M.n;
-  jacobian_into(A.M, V);
-  double a[TILE][TILE];
-  assemble<TILE>(P, JLds{V.jv}, 1.0, A.b.mass, true, L.lds, a);
-  factor_cell<TILE>(P, T, L, a);
-  double* vec = L.lds + P.vec;
-  for (int q = threadIdx.x; q < P.c + P.mpad; q += NT) vec[q] = q < n ? rhs[P.perm[q]] : 0.0;
-  solve_cell<TILE>(P, T, L, a);
-  for (int q = threadIdx.x; q < n; q += NT) out[P.perm[q]] = vec[q];
-  __syncthreads();
-}
-
-__device__ inline double weighted_term(double z, double d, double atol, double rtol) {
-  const double sc = atol + rtol * fmax(fabs(z), fabs(z + d));
-  return sc > 0.0 ? fabs(d) / sc : (d == 0.0 ? 0.0 : INFINITY);
-}
-
-// project (fastode/solvers.py:100-178) for one group, on the device: damped
-// Newton on the algebraic rows with group-wide maxima, the factorization
-// recomputed cell by cell.  The group's Newton corrections (algebraic rows)
-// live in work, its saved state in the second half of work.  Returns true
-// when converged; a failure restores the state.
-template <int TILE>
-__device__ __forceinline__ bool project_group(const BatchArgs& A, const Tables& T, const EngineLds& L, const CellLds& V,
-                              fode_group_ctrl& gs, int c0, int cells) {
-  const fode_batch_desc& B = A.b;
-  const int n = A.M.n, tid = threadIdx.x;
-  const double* mass = B.mass;
-  double* Yg = B.Y + (size_t)c0 * n;
-  double* Dg = B.work + (size_t)c0 * n;
-  double* Og = B.work + ((size_t)B.C + c0) * n;
-  constexpr int MAX_UPDATES = 10, MAX_BACKTRACKS = 24;
-  constexpr double TOL = 0.1, FACTOR = 0.5, SIGMA = 1.0e-4;
-  auto count = [&](int which) {
-    if (tid == 0) gs.counts[which] += 1;
-  };
-  bool saved = false;
-  for (int update = 0; update <= MAX_UPDATES; ++update) {
-    double gmax = 0.0, bad = 0.0;
-    for (int c = 0; c < cells; ++c) {
-      load_cell(A, V, c0 + c);
-      residual_into(A, V, V.y, V.f0);
-      for (int i = tid; i < n; i += NT)
-        if (mass[i] == 0.0) {
-          const double gi = V.f0[i];
-          if (!isfinite(gi)) bad = 1.0;
-          gmax = fmax(gmax, fabs(gi));
-        }
-    }
-    count(FODE_C_RHS);
-    bad = block_max_all(bad, V.red);
-    gmax = block_max_all(gmax, V.red);
-    if (bad != 0.0) break;
-    count(FODE_C_PROJ_PASSES);
-    if (gmax == 0.0) return true;
-    count(FODE_C_JAC);
-    count(FODE_C_LU);
-    count(FODE_C_SOLVES);
-    double q = 0.0;
-    bad = 0.0;
-    for (int c = 0; c < cells; ++c) {
-      load_cell(A, V, c0 + c);
-      residual_into(A, V, V.y, V.f0);
-      alg_solve<TILE>(A, T, L, V, V.f0, V.fr);
-      for (int i = tid; i < n; i += NT)
-        if (mass[i] == 0.0) {
-          const double d = V.fr[i];
-          Dg[(size_t)c * n + i] = d;
-          if (!isfinite(d)) bad = 1.0;
-          q = fmax(q, weighted_term(V.y[i], d, B.atol[i], B.rtol));
-        }
-    }
-    bad = block_max_all(bad, V.red);
-    q = block_max_all(q, V.red);
-    if (bad != 0.0) break;
-    if (q <= TOL) return true;
-    if (update == MAX_UPDATES) break;
-    if (!saved) {
-      for (long e = tid; e < (long)cells * n; e += NT)
-        if (mass[e % n] == 0.0) Og[e] = Yg[e];
-      saved = true;
-    }
-    double step = 1.0, taken = -1.0;
-    for (int bt = 0; bt <= MAX_BACKTRACKS; ++bt) {
-      const double used = step;
-      step *= FACTOR;
-      double cbad = 0.0, gbad = 0.0, gcmax = 0.0;
-      for (int c = 0; c < cells; ++c) {
-        load_cell(A, V, c0 + c);
-        for (int i = tid; i < n; i += NT) {
-          double v = V.y[i];
-          if (mass[i] == 0.0) {
-            v = v + used * Dg[(size_t)c * n + i];
-            if (!isfinite(v)) cbad = 1.0;
-          }
-          V.w[i] = v;
-        }
-        __syncthreads();
-        residual_into(A, V, V.w, V.f0);
-        for (int i = tid; i < n; i += NT)
-          if (mass[i] == 0.0) {
-            if (!isfinite(V.f0[i])) gbad = 1.0;
-            gcmax = fmax(gcmax, fabs(V.f0[i]));
-          }
-      }
-      cbad = block_max_all(cbad, V.red);
-      gbad = block_max_all(gbad, V.red);
-      gcmax = block_max_all(gcmax, V.red);
-      if (cbad != 0.0) continue;
-      count(FODE_C_RHS);
-      count(FODE_C_PROJ_TRIALS);
-      if (gbad != 0.0) continue;
-      if (gcmax == 0.0) {
-        taken = used;
-        break;
-      }
-      count(FODE_C_SOLVES);
-      double qt = 0.0, tbad = 0.0;
-      for (int c = 0; c < cells; ++c) {
-        load_cell(A, V, c0 + c);
-        for (int i = tid; i < n; i += NT)
-          V.w[i] = mass[i] == 0.0 ? V.y[i] + used * Dg[(size_t)c * n + i] : V.y[i];
-        __syncthreads();
-        residual_into(A, V, V.w, V.f0);
-        alg_solve<TILE>(A, T, L, V, V.f0, V.fr);     // the factorization at Y, as LU.solve reuses it
-        for (int i = tid; i < n; i += NT)
-          if (mass[i] == 0.0) {
-            const double d = V.fr[i];
-            if (!isfinite(d)) tbad = 1.0;
-            qt = fmax(qt, weighted_term(V.w[i], d, B.atol[i], B.rtol));
-          }
-      }
-      tbad = block_max_all(tbad, V.red);
-      qt = block_max_all(qt, V.red);
-      if (tbad != 0.0) continue;
-      if (qt <= TOL || qt < (1.0 - SIGMA * used) * q) {
-        taken = used;
-        break;
-      }
-    }
-    if (taken < 0.0) break;
-    for (long e = tid; e < (long)cells * n; e += NT)
-      if (mass[e % n] == 0.0) Yg[e] = Yg[e] + taken * Dg[e];
-    __syncthreads();
-  }
-  if (saved) {
-    for (long e = tid; e < (long)cells * n; e += NT)
-      if (mass[e % n] == 0.0) Yg[e] = Og[e];
-  }
-  __syncthreads();
-  return false;
-}
-
-// DAE: the tableau is constrained (projection at the start of each call);
-// a separate instantiation keeps the projection's registers out of the
-// unconstrained kernel.
-template <int TILE, bool DAE>
-__global__ __launch_bounds__(NT) void k_batched(BatchArgs A) {
-  extern __shared__ double smem[];
-  const int n = A.M.n, S = A.p.stages;
-  const Tables T = A.lay.tables > 0 ? stage_tables(A.P, reinterpret_cast<uint32_t*>(smem)) : A.P.t;
-  const EngineLds L = carve(smem + A.lay.engine, A.P);
-  const CellLds V = carve_cell(smem, A.lay, n, A.M.R, S);
-  __shared__ fode_group_ctrl gs;
-  const int tid = threadIdx.x;
-  const fode_batch_desc& B = A.b;
-  for (;;) {
-    if (tid == 0) {
-      const int idx = atomicAdd(B.queue, 1);
-      int grp = idx;
-      if (idx < B.ngroups && B.order) {
-        grp = B.order[idx];
-        if (grp < 0 || grp >= B.ngroups) grp = idx;   // not a permutation: ignore it
-      }
-      V.flags[0] = grp;
-    }
-    __syncthreads();
-    const int group = V.flags[0];
-    __syncthreads();
-    if (group >= B.ngroups) break;
-    const int c0 = group * B.group_size;
-    const int cells = min(B.group_size, B.C - c0);
-    if (tid == 0) {
-      gs = B.g[group];
-      gs.dt = B.dt;
-      ctrl_init(A.p, gs);
-    }
-    __syncthreads();
-    // consistent initialization of constrained systems (solvers.py:322-325):
-    // a failure returns the restored state, unclamped, with Memory untouched
-    bool consistent = true;
-    if constexpr (DAE) consistent = project_group<TILE>(A, T, L, V, gs, c0, cells);
-    if (tid == 0) {
-      if (consistent) {
-        ctrl_outer(A.p, gs);
-      } else {
-        gs.status = FODE_PROJECTION_FAILED;
-        gs.action = FODE_FINISH;
-      }
-    }
-    __syncthreads();
-    int nrec = 0;   // accepted steps recorded (thread 0)
-    while (gs.action != FODE_FINISH) {
-      double sum = 0.0;
-      for (int c = 0; c < cells; ++c) {
-        double* ynew = B.work + (size_t)(c0 + c) * n;
-        const double cs = cell_attempt<TILE>(A, T, L, V, gs, c0 + c, ynew);
-        sum = sum + cs;
-      }
-      if (tid == 0) {
-        const double h_used = gs.H;
-        ctrl_after(A.p, gs, err_from_sum(sum, (double)cells * n));
-        if (gs.accepted && B.hlog && nrec < B.hcap) B.hlog[(size_t)group * B.hcap + nrec++] = h_used;
-      }
-      __syncthreads();
-      if (gs.accepted) {
-        const long N = (long)cells * n;
-        double* Y = B.Y + (size_t)c0 * n;
-        const double* W = B.work + (size_t)c0 * n;
-        for (long i = tid; i < N; i += NT) Y[i] = W[i];
-      }
-      __syncthreads();
-    }
-    // clamp (fastode/solvers.py:89-97) and record the call's status
-    if (gs.status != FODE_PROJECTION_FAILED) {
-      const long N = (long)cells * n;
-      double* Y = B.Y + (size_t)c0 * n;
-      for (long i = tid; i < N; i += NT) {
-        if (A.p.constrained && B.mass[i % n] != 1.0) continue;
-        Y[i] = fmax(Y[i], 0.0);
-      }
-    }
-    if (tid == 0) {
-      if (gs.status != FODE_CONVERGED) {
-        if (gs.fails == 0) gs.first_fail = (int)gs.counts[FODE_C_CALLS] - 1;
-        gs.fails += 1;
-      }
-      B.g[group] = gs;
-      if (B.nacc) B.nacc[group] = nrec;
-    }
-    __syncthreads();
-  }
-}
 
 __global__ void k_reset_queue(int* q) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *q = 0;
-}
-
-template <int TILE, bool DAE>
-int launch_batched_as(BatchArgs A, int blocks_per_cu, hipStream_t s) {
-  const void* kern = (const void*)k_batched<TILE, DAE>;
-  const size_t bytes = (size_t)A.lay.total * sizeof(double);
-  int dev = 0, cus = 0, maxlds = 0;
-  (void)hipGetDevice(&dev);
-  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  (void)hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
-  if (bytes + sizeof(fode_group_ctrl) > (size_t)maxlds)
-    return bfail("fode_batched_interval: the plan needs " + std::to_string(bytes) + " bytes of LDS, above " +
-                 std::to_string(maxlds));
-  hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-  if (e != hipSuccess) return bfail(std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
-  int occ = 0;
-  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, NT, bytes);
-  if (e != hipSuccess || occ < 1) occ = 1;
-  const int per_cu = blocks_per_cu > 0 ? blocks_per_cu : occ;
-  int blocks = cus * per_cu;
-  if (blocks > A.b.ngroups) blocks = A.b.ngroups;
-  if (blocks < 1) blocks = 1;
-  hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, s, A.b.queue);
-  hipLaunchKernelGGL((k_batched<TILE, DAE>), dim3(blocks), dim3(NT), bytes, s, A);
-  e = hipGetLastError();
-  if (e != hipSuccess) return bfail(std::string("k_batched: ") + hipGetErrorString(e));
-  return 0;
-}
-
-template <int TILE>
-int launch_batched(BatchArgs A, int blocks_per_cu, hipStream_t s) {
-  return A.p.constrained ? launch_batched_as<TILE, true>(A, blocks_per_cu, s)
-                         : launch_batched_as<TILE, false>(A, blocks_per_cu, s);
 }
 
 }  // namespace
@@ -465,12 +104,10 @@ int fode_batched_interval(void* plan, void* mech, const fode_ctrl_params* p, con
   A.b = *b;
   A.lay = choose_layout(A.P, A.M, p->stages, b->options);
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-  switch (A.P.tile) {
-#define X(T) case T: return launch_batched<T>(A, blocks_per_cu, s);
-    X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10)
-#undef X
-  }
-  return bfail("fode_batched_interval: unsupported tile");
+  if (A.P.tile < 1 || A.P.tile > 10) return bfail("fode_batched_interval: unsupported tile");
+  hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, s, A.b.queue);
+  return A.lay.tables > 0 ? launch_batched_tiles<true>(A, blocks_per_cu, s)
+                          : launch_batched_unstaged(A, blocks_per_cu, s);
 }
 
 }  // extern "C"
