@@ -311,16 +311,21 @@ __device__ inline void schur(const PlanDev& P, const Tables& T, const double* ld
       const int s = ti * TILE + tj;
       row |= ((mw[s >> 5] >> (s & 31)) & 1u) << tj;
     }
-    // one branch per tile row: a row's loads issue together (S has
-    // sch_depth + 1 rows, so the unmasked reads stay inside it)
+    // one branch per tile row; a slot's sum index is j plus the row's set
+    // bits before it, so the row's loads are independent and issue together
+    // (S has sch_depth + 1 rows, so the unmasked reads stay inside it)
     if (row) {
+      double sv[TILE];
+#pragma unroll
+      for (int tj = 0; tj < TILE; ++tj)
+        sv[tj] = S[(j + __builtin_popcount(row & ((1u << tj) - 1u))) * NT + tid];
 #pragma unroll
       for (int tj = 0; tj < TILE; ++tj) {
         const uint32_t bit = (row >> tj) & 1u;
-        const long long raw = __double_as_longlong(S[j * NT + tid]);
+        const long long raw = __double_as_longlong(sv[tj]);
         a[ti][tj] = a[ti][tj] + __longlong_as_double(raw & -(long long)bit);
-        j += bit;
       }
+      j += __builtin_popcount(row);
     }
   }
 }
