@@ -279,13 +279,15 @@ __device__ inline void schur(const PlanDev& P, const Tables& T, const double* ld
   const int q1 = P.sch_base[tid + 1];
   double acc = 0.0;
   int j = 0;
+  // branch-free: the running sum is stored at its slot after every pair
+  // (the closing pair's store is the last one there), then the slot index
+  // advances and the sum restarts when the pair closes it
   auto step = [&](uint32_t w, double x, double y) {
     acc = fma(-x, y, acc);
-    if (w >> 31) {
-      S[j * NT + tid] = acc;
-      ++j;
-      acc = 0.0;
-    }
+    S[j * NT + tid] = acc;
+    const int close = (int)(w >> 31);
+    j += close;
+    acc = close ? 0.0 : acc;
   };
   // groups of 4 pairs (16-byte aligned; the zero padding after the last
   // close only feeds a sum that is never stored)
