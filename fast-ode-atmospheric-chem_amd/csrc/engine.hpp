@@ -477,14 +477,30 @@ __device__ inline void diag_inverses(const double (&a)[TILE][TILE], const Engine
   }
 }
 
+// x[lane of the even row] + x[lane of the odd row] of each pair of DPP rows
+// (lanes 16 apart), and of the two wave halves (lanes 32 apart): gfx950's
+// permlane swaps give each lane both operands in registers, low one first,
+// so both lanes of a pair form the same sum.
+__device__ inline double pair_sum16(double x) {
+  const unsigned lo = (unsigned)__double2loint(x), hi = (unsigned)__double2hiint(x);
+  const auto l = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+  const auto h = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+  return __hiloint2double((int)h[0], (int)l[0]) + __hiloint2double((int)h[1], (int)l[1]);
+}
+__device__ inline double pair_sum32(double x) {
+  const unsigned lo = (unsigned)__double2loint(x), hi = (unsigned)__double2hiint(x);
+  const auto l = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+  const auto h = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+  return __hiloint2double((int)h[0], (int)l[0]) + __hiloint2double((int)h[1], (int)l[1]);
+}
+
 // Sum over the 16 grid rows of x (the threads of column tc): first over
-// the wave's four rows (lanes 16 and 32 apart) by lane shuffles, then over
-// the four waves' partials through p (one barrier; 4 LDS reads a thread, not
-// 16).  The same bits in every thread of the column.
+// the wave's four rows by register permutes, then over the four waves'
+// partials through p (one barrier; 4 LDS reads a thread).  The same bits in
+// every thread of the column.
 __device__ inline double colsum16(double x, double* p) {
   const int tid = threadIdx.x, lane = tid & 63;
-  x += __shfl_xor(x, 16, 64);
-  x += __shfl_xor(x, 32, 64);
+  x = pair_sum32(pair_sum16(x));
   if (lane < G) p[(tid >> 6) * G + lane] = x;
   __syncthreads();
   const int tc = tid % G;
