@@ -67,17 +67,38 @@ AdjArgs make_args(const PlanDev& P, const PlanDev& PT, const MechDev& M, int S) 
   return A;
 }
 
-// (S^T v)_j
+// (S^T v)_j; four terms' loads in flight per round, the sum in term order
 __device__ inline double st_dot(const MechDev& M, const double* v, int j) {
   double g = 0.0;
-  for (int q = M.st_ptr[j]; q < M.st_ptr[j + 1]; ++q) g = g + M.st_coef[q] * v[M.st_sp[q]];
+  int q = M.st_ptr[j];
+  const int qe = M.st_ptr[j + 1];
+  for (; q + 4 <= qe; q += 4) {
+    const int s0 = M.st_sp[q], s1 = M.st_sp[q + 1], s2 = M.st_sp[q + 2], s3 = M.st_sp[q + 3];
+    const double c0 = M.st_coef[q], c1 = M.st_coef[q + 1], c2 = M.st_coef[q + 2], c3 = M.st_coef[q + 3];
+    const double v0 = v[s0], v1 = v[s1], v2 = v[s2], v3 = v[s3];
+    g = g + c0 * v0;
+    g = g + c1 * v1;
+    g = g + c2 * v2;
+    g = g + c3 * v3;
+  }
+  for (; q < qe; ++q) g = g + M.st_coef[q] * v[M.st_sp[q]];
   return g;
 }
 
-// sum of the slot values of species i
+// sum of the slot values of species i (same rounds)
 __device__ inline double gather_slots(const MechDev& M, const double* phi, int i) {
   double s = 0.0;
-  for (int q = M.sp_ptr[i]; q < M.sp_ptr[i + 1]; ++q) s = s + phi[M.sp_jp[q]];
+  int q = M.sp_ptr[i];
+  const int qe = M.sp_ptr[i + 1];
+  for (; q + 4 <= qe; q += 4) {
+    const double p0 = phi[M.sp_jp[q]], p1 = phi[M.sp_jp[q + 1]];
+    const double p2 = phi[M.sp_jp[q + 2]], p3 = phi[M.sp_jp[q + 3]];
+    s = s + p0;
+    s = s + p1;
+    s = s + p2;
+    s = s + p3;
+  }
+  for (; q < qe; ++q) s = s + phi[M.sp_jp[q]];
   return s;
 }
 
@@ -87,6 +108,19 @@ __device__ inline double rate_hess(const MechDev& M, const double* y, const doub
                                    int j, double g, double* phi) {
   const int P = M.P, n = M.n;
   const int* rj = M.react + j * P;
+  if (P == 2) {   // straight-line form of the loops below (same operations)
+    const int r0 = rj[0], r1 = rj[1];
+    const bool h0 = r0 < n, h1 = r1 < n;
+    const double kj = k[j];
+    const double y0 = h0 ? y[r0] : 1.0, y1 = h1 ? y[r1] : 1.0;
+    const double v0 = h0 ? v[r0] : 0.0, v1 = h1 ? v[r1] : 0.0;
+    double hk = 0.0;
+    if (h0) hk = hk + (kj * y1) * v0;
+    if (h1) hk = hk + (kj * y0) * v1;
+    phi[j * 2] = g * ((h0 && h1) ? 0.0 + kj * v1 : 0.0);
+    phi[j * 2 + 1] = g * ((h0 && h1) ? 0.0 + kj * v0 : 0.0);
+    return hk;
+  }
   double hk = 0.0;
   for (int p = 0; p < P; ++p)
     if (rj[p] < n) hk = hk + drate(M, y, k, j, p) * v[rj[p]];
