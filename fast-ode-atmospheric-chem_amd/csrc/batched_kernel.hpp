@@ -50,7 +50,10 @@ __device__ __forceinline__ double cell_attempt(const BatchArgs& A, const Tables&
   const double* yg = A.b.Y + (size_t)cell * n;
   const double* kg = A.b.k + (size_t)cell * A.b.k_stride;
   for (int i = tid; i < n; i += NT) V.y[i] = yg[i];
-  for (int j = tid; j < M.R; j += NT) V.kc[j] = kg[j] * A.b.kmul[j];
+  // one shared row of rate constants (k_stride 0): V.kc was formed once for
+  // the launch (the cell computations only read it)
+  if (A.b.k_stride != 0)
+    for (int j = tid; j < M.R; j += NT) V.kc[j] = kg[j] * A.b.kmul[j];
   __syncthreads();
   cell_stages<TILE>(A.P, T, M, A.p, A.b.mass, L, V, g.H, g.alpha, ynew_out);
   return cell_err_sum(V.y, V.w, V.fr, A.b.atol, A.b.rtol, n, V.red);
@@ -265,6 +268,8 @@ __global__ __launch_bounds__(NT) void k_batched(BatchArgs A) {
   __shared__ fode_group_ctrl gs;
   const int tid = threadIdx.x;
   const fode_batch_desc& B = A.b;
+  if (B.k_stride == 0)
+    for (int j = tid; j < A.M.R; j += NT) V.kc[j] = B.k[j] * B.kmul[j];
   for (;;) {
     if (tid == 0) {
       const int idx = atomicAdd(B.queue, 1);

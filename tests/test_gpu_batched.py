@@ -91,10 +91,13 @@ def test_batched_matches_golden_mech100_constrained(graph):
     assert worst <= 1.0
 
 
-def test_batched_constrained_equals_plain_bitwise():
+@pytest.mark.parametrize("stage_tables", [True, False])
+def test_batched_constrained_equals_plain_bitwise(stage_tables):
     """The constrained workload on a ragged batch (40 cells: 32 + 8), the
     initial state pushed off its constraints so that the projection takes a
-    Newton update: batched and plain agree bit for bit, Counts included."""
+    Newton update: batched and plain agree bit for bit, Counts included, with
+    the plan's tables staged in LDS and read from memory (two kernel
+    instantiations)."""
     import fastode_amd as fa
     from fastode_amd import workloads as W
     from fastode_amd.system import ChemSystem
@@ -103,7 +106,8 @@ def test_batched_constrained_equals_plain_bitwise():
     Y0[:, wl.mech.constraint_rows] *= 1.0 + 1e-3 * np.arange(1, 41)[:, None]
     k = static_k(wl, np.arange(40))
     system = ChemSystem(wl.mech, k, mass=wl.mass)
-    run = fa.BatchedAlgorithm(fa.RODAS3).runner(system, Y0, wl.dt, wl.atol, wl.rtol, graph=False)
+    run = fa.BatchedAlgorithm(fa.RODAS3, stage_tables=stage_tables).runner(system, Y0, wl.dt, wl.atol, wl.rtol,
+                                                                          graph=False)
     alg = fa.Algorithm(fa.RODAS3)
     mems = [alg.new_memory() for _ in range(2)]
     cnt = [fa.Counts() for _ in range(2)]
