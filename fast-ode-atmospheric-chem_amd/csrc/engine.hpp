@@ -23,7 +23,7 @@ struct EngineLds {
   double* lds;    // [lds_len]: sparse entries, then the solve vector (c + mpad)
   double* lbuf;   // [4*mpad]   pivot columns, two per pair, double-buffered
   double* ubuf;   // [4*mpad]   pivot rows
-  double* rbuf;   // [mpad]     pivot reciprocals
+  double* rbuf;   // [mpad]     reciprocals of the first pivot of each pair
   double* pb;     // [2*NT]     block-solve partial products
   double* scr;    // [tile*NT]  diagonal blocks being inverted
   double* dL;     // [tile*NT]  inverse diagonal blocks of L, transposed
@@ -365,6 +365,7 @@ template <int TILE, int B>
 __device__ inline void factor_block(double (&a)[TILE][TILE], double* lbuf, double* ubuf,
                                     double* rcs, int tr, int tc) {
   constexpr int MP = G * TILE;
+  double rcown = 0.0;
 #pragma nounroll
   for (int kk = 0; kk < G; kk += 2) {
     const int k0 = B * G + kk;
@@ -377,7 +378,9 @@ __device__ inline void factor_block(double (&a)[TILE][TILE], double* lbuf, doubl
     const double l10 = c0[B * G + kk + 1] * rc0;   // multiplier of row k1 for pivot k0
     const double u01 = r0[B * G + kk + 1];         // row k0 at column k1
     const double rc1 = 1.0 / fma(-l10, u01, r1[B * G + kk + 1]);
-    if (tr == kk + 1 && tc == kk + 1) rcs[k0 + 1] = rc1;
+    // the reciprocal of this thread's own pivot column, for the scaling below
+    rcown = (tc == kk) ? rc0 : rcown;
+    rcown = (tc == kk + 1) ? rc1 : rcown;
     double u0[TILE], u1[TILE];
 #pragma unroll
     for (int tj = B; tj < TILE; ++tj) {
@@ -407,7 +410,7 @@ __device__ inline void factor_block(double (&a)[TILE][TILE], double* lbuf, doubl
     __syncthreads();
   }
   // the L entries of tile column B (below the pivots' diagonal)
-  const double rcb = rcs[B * G + tc];
+  const double rcb = rcown;
 #pragma unroll
   for (int ti = B; ti < TILE; ++ti) {
     if (ti == B) {
