@@ -332,6 +332,17 @@ __device__ inline void schur(const PlanDev& P, const Tables& T, const double* ld
   }
 }
 
+// 1/x for the core's pivots (finite, nonzero, normal range): the hardware
+// reciprocal refined by two Newton-Raphson steps r += r (1 - x r), the
+// residual formed by fma -- a few instructions against the IEEE division's
+// scaled sequence, within an ulp of 1/x.
+__device__ inline double pivot_rcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(r, fma(-x, r, 1.0), r);
+  r = fma(r, fma(-x, r, 1.0), r);
+  return r;
+}
+
 // Publish the current column and row of pivots k0 = NB*G + kk and k0 + 1
 // (updated through pivot k0 - 1) into pair buffer nb, and the reciprocal of
 // pivot k0 (final at that point) at rcs[k0].
@@ -349,7 +360,7 @@ __device__ inline void publish_pair(const double (&a)[TILE][TILE], int kk, int n
   if (dr == 0 || dr == 1) {
 #pragma unroll
     for (int tj = NB; tj < TILE; ++tj) r[dr * MP + tj * G + tc] = a[NB][tj];
-    if (dr == 0 && dc == 0) rcs[NB * G + kk] = 1.0 / a[NB][NB];
+    if (dr == 0 && dc == 0) rcs[NB * G + kk] = pivot_rcp(a[NB][NB]);
   }
 }
 
@@ -377,7 +388,7 @@ __device__ inline void factor_block(double (&a)[TILE][TILE], double* lbuf, doubl
     const double rc0 = rcs[k0];
     const double l10 = c0[B * G + kk + 1] * rc0;   // multiplier of row k1 for pivot k0
     const double u01 = r0[B * G + kk + 1];         // row k0 at column k1
-    const double rc1 = 1.0 / fma(-l10, u01, r1[B * G + kk + 1]);
+    const double rc1 = pivot_rcp(fma(-l10, u01, r1[B * G + kk + 1]));
     // the reciprocal of this thread's own pivot column, for the scaling below
     rcown = (tc == kk) ? rc0 : rcown;
     rcown = (tc == kk + 1) ? rc1 : rcown;
