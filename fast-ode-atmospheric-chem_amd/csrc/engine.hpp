@@ -77,8 +77,19 @@ __device__ inline double rowsum16(double x) {
   return x;
 }
 
+// 1/x for pivots (finite, nonzero, normal range): the hardware
+// reciprocal refined by two Newton-Raphson steps r += r (1 - x r), the
+// residual formed by fma -- a few instructions against the IEEE division's
+// scaled sequence, within an ulp of 1/x.
+__device__ inline double pivot_rcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(r, fma(-x, r, 1.0), r);
+  r = fma(r, fma(-x, r, 1.0), r);
+  return r;
+}
+
 // Level-scheduled dot-product tasks over LDS:
-//   lds[t] = (lds[t] - sum_q lds[a_q] * lds[b_q]) [/ lds[d]]
+//   lds[t] = (lds[t] - sum_q lds[a_q] * lds[b_q]) [* 1/lds[d]]
 // Tasks of one stage are independent; a barrier follows every stage.
 
 // One task per thread: the pairs in order, four pairs' loads in flight at a
@@ -103,7 +114,7 @@ __device__ inline void task_single(const uint2 tk, const uint32_t* pairs, double
     const uint32_t w = pairs[q];
     acc = fma(-lds[w >> 16], lds[w & 0xFFFFu], acc);
   }
-  if (div != NO_DIV) acc = acc / lds[div];
+  if (div != NO_DIV) acc = acc * pivot_rcp(lds[div]);
   lds[tgt] = acc;
 }
 
@@ -111,7 +122,7 @@ __device__ inline void task_single(const uint2 tk, const uint32_t* pairs, double
 // in the solves): lane j of the quad sums the pairs q0 + j, q0 + j + 4, ...
 // in order, the four partial sums are combined by DPP quad permutes (the
 // same bits in all four lanes), and lane 0 writes
-//   lds[t] = (lds[t] - ((s0 + s1) + (s2 + s3))) [/ lds[d]].
+//   lds[t] = (lds[t] - ((s0 + s1) + (s2 + s3))) [* 1/lds[d]].
 // Called with every lane of the workgroup active (the DPP reads them all).
 __device__ inline void task_quad(const uint2* tasks, int t, int e, const uint32_t* pairs, double* lds) {
   const int j = threadIdx.x & 3;
@@ -144,7 +155,7 @@ __device__ inline void task_quad(const uint2* tasks, int t, int e, const uint32_
   part += dpp_mov<0x4E>(part);   // quad_perm [2, 3, 0, 1]
   if (live && j == 0) {
     double acc = lds[tgt] - part;
-    if (div != NO_DIV) acc = acc / lds[div];
+    if (div != NO_DIV) acc = acc * pivot_rcp(lds[div]);
     lds[tgt] = acc;
   }
 }
@@ -332,17 +343,6 @@ __device__ inline void schur(const PlanDev& P, const Tables& T, const double* ld
   }
 }
 
-// 1/x for the core's pivots (finite, nonzero, normal range): the hardware
-// reciprocal refined by two Newton-Raphson steps r += r (1 - x r), the
-// residual formed by fma -- a few instructions against the IEEE division's
-// scaled sequence, within an ulp of 1/x.
-__device__ inline double pivot_rcp(double x) {
-  double r = __builtin_amdgcn_rcp(x);
-  r = fma(r, fma(-x, r, 1.0), r);
-  r = fma(r, fma(-x, r, 1.0), r);
-  return r;
-}
-
 // Publish the current column and row of pivots k0 = NB*G + kk and k0 + 1
 // (updated through pivot k0 - 1) into pair buffer nb, and the reciprocal of
 // pivot k0 (final at that point) at rcs[k0].
@@ -480,7 +480,7 @@ __device__ inline void diag_inverses(const double (&a)[TILE][TILE], const Engine
           double v = (r == col) ? 1.0 : 0.0;
 #pragma unroll
           for (int s = r + 1; s < G; ++s) v = fma(-M[r * G + s], x[s], v);
-          x[r] = v / M[r * G + r];
+          x[r] = v * pivot_rcp(M[r * G + r]);
         }
       }
       // inverse entry (r, col) goes to col*G + r: thread (tr, tc) reads (tc, tr)
