@@ -164,3 +164,27 @@ def test_batched_equals_plain_bitwise_and_graph_replays_match_eager():
     assert np.array_equal(re["counts"], rg["counts"])
     for gi in range(3):
         assert re["h_next"][gi] == mems[gi].h_next
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_shared_rate_row_equals_per_cell_rows(graph):
+    """One shared rate-constant row (k_stride 0, scaled once per launch) and
+    the same row repeated per cell give the same bits, over intervals with
+    different multipliers (the per-launch scaling follows each interval's)."""
+    import fastode_amd as fa
+    from fastode_amd import workloads as W
+    from fastode_amd.system import ChemSystem
+    wl = W.diurnal200(C=40)     # photolysis multipliers change every interval
+    shared = ChemSystem(wl.mech, wl.k_base[None, :])
+    per_cell = ChemSystem(wl.mech, np.broadcast_to(wl.k_base, (40, wl.R)).copy())
+    balg = fa.BatchedAlgorithm(fa.ROS3)
+    a = balg.runner(shared, wl.Y0, wl.dt, wl.atol, wl.rtol, graph=graph)
+    b = balg.runner(per_cell, wl.Y0, wl.dt, wl.atol, wl.rtol, graph=graph)
+    for it in range(3):
+        km = kmul_for(wl, it + 3)
+        if it:
+            assert not np.array_equal(km, kmul_for(wl, it + 2))
+        a.step(km)
+        b.step(km)
+        assert np.array_equal(a.Y.cpu().numpy(), b.Y.cpu().numpy()), it
+    assert np.array_equal(a.memory.counts(), b.memory.counts())
