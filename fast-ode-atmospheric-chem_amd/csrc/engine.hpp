@@ -526,14 +526,21 @@ __device__ inline double colsum16(double x, double* p) {
 template <int TILE>
 __device__ inline void core_solve(const double (&a)[TILE][TILE], const EngineLds& L, double* v) {
   const int tid = threadIdx.x, tr = tid / G, tc = tid % G;
-  double acc[TILE];
+  // each step's inverse entry and right-hand side are read up front (a block
+  // of v is written only by its own step, after its reads), so the steps'
+  // chains start without an LDS round trip
+  double acc[TILE], dv[TILE], vv[TILE];
 #pragma unroll
-  for (int I = 0; I < TILE; ++I) acc[I] = 0.0;
+  for (int I = 0; I < TILE; ++I) {
+    acc[I] = 0.0;
+    dv[I] = L.dL[I * NT + tid];
+    vv[I] = v[I * G + tr];
+  }
 #pragma unroll
   for (int J = 0; J < TILE; ++J) {
-    const double d = L.dL[J * NT + tid];
+    const double d = dv[J];
     const double s = rowsum16(acc[J]);
-    const double r = v[J * G + tr] - s;
+    const double r = vv[J] - s;
     const double y = colsum16(d * r, L.pb + (J & 1) * NT);
     if (tr == 0) v[J * G + tc] = y;
 #pragma unroll
@@ -541,12 +548,16 @@ __device__ inline void core_solve(const double (&a)[TILE][TILE], const EngineLds
   }
   __syncthreads();
 #pragma unroll
-  for (int I = 0; I < TILE; ++I) acc[I] = 0.0;
+  for (int I = 0; I < TILE; ++I) {
+    acc[I] = 0.0;
+    dv[I] = L.dU[I * NT + tid];
+    vv[I] = v[I * G + tr];
+  }
 #pragma unroll
   for (int J = TILE - 1; J >= 0; --J) {
-    const double d = L.dU[J * NT + tid];
+    const double d = dv[J];
     const double s = rowsum16(acc[J]);
-    const double r = v[J * G + tr] - s;
+    const double r = vv[J] - s;
     const double x = colsum16(d * r, L.pb + (J & 1) * NT);
     if (tr == 0) v[J * G + tc] = x;
 #pragma unroll
