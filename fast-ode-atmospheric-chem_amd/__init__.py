@@ -20,15 +20,18 @@ __all__ = ["ROS3", "RODAS3", "Tableau", "Counts", "Memory"]
 def __getattr__(name):
     # Device-facing pieces load the HIP library on first use, so that the
     # host-only modules (mechanism, symbolic, workloads) import without it.
-    if name in ("Algorithm", "project", "clamp"):
+    if name in ("Algorithm", "RestartingLoop", "BackwardEuler", "project", "clamp", "fixed_steps"):
         from . import solvers
         return getattr(solvers, name)
     if name in ("ChemSystem", "DenseSystem"):
         from . import system
         return getattr(system, name)
-    if name in ("BatchedAlgorithm",):
+    if name in ("BatchedAlgorithm", "BatchedRestartingLoop", "BatchedBackwardEuler"):
         from . import batched
         return getattr(batched, name)
+    if name in ("TS1", "Cells", "Scenario", "run_scenario", "run_scenario_batched", "score"):
+        from . import ts1
+        return getattr(ts1, name)
     if name in ("BatchedAdjoint", "integrate"):
         from . import adjoint
         return getattr(adjoint, name)

@@ -23,11 +23,13 @@ if not LIB_PATH.exists():
 lib = ct.CDLL(str(LIB_PATH))
 
 NCOUNTS = 13
-RUNNING, CONVERGED, TOO_MANY_STEPS, STEP_TOO_SMALL, NAN_OR_INF, PROJECTION_FAILED = range(6)
-FINISH, NEW_STEP, RETRY = range(3)
+RUNNING, CONVERGED, TOO_MANY_STEPS, STEP_TOO_SMALL, NAN_OR_INF, PROJECTION_FAILED, ACCEPTED_UNCONVERGED = range(7)
+FINISH, NEW_STEP, RETRY, NEWTON = range(4)
+LOOP_ALGORITHM, LOOP_RESTARTING, LOOP_BACKWARD_EULER = range(3)
 STATUS = {CONVERGED: "converged", TOO_MANY_STEPS: "too many steps",
           STEP_TOO_SMALL: "step too small", NAN_OR_INF: "nan or inf",
-          PROJECTION_FAILED: "projection failed", RUNNING: "running"}
+          PROJECTION_FAILED: "projection failed", RUNNING: "running",
+          ACCEPTED_UNCONVERGED: "accepted unconverged"}
 
 # fode_group_ctrl as a NumPy record (232 bytes, 8-byte aligned fields).
 CTRL_DTYPE = np.dtype([
@@ -52,7 +54,9 @@ class CtrlParams(ct.Structure):
                 ("carry_memory", ct.c_int32), ("max_steps", ct.c_int32), ("stages", ct.c_int32),
                 ("n_new_forcing", ct.c_int32), ("constrained", ct.c_int32),
                 ("a", ct.c_double * 16), ("c", ct.c_double * 16), ("m", ct.c_double * 4),
-                ("e", ct.c_double * 4), ("new_forcing", ct.c_int32 * 4)]
+                ("e", ct.c_double * 4), ("new_forcing", ct.c_int32 * 4),
+                ("loop", ct.c_int32), ("step_change_rows", ct.c_int32), ("max_iterations", ct.c_int32),
+                ("n_reductions", ct.c_int32), ("reductions", ct.c_double * 8), ("small", ct.c_double)]
 
 
 class MechDesc(ct.Structure):
@@ -119,6 +123,9 @@ _SIGS = {
     "fode_stage_final": (i32, [P(CtrlParams), i32, i32, vp, P(vp), vp, vp, vp]),
     "fode_norm_control": (i32, [P(CtrlParams), vp, i32, i32, vp, vp, vp, vp, ct.c_double, vp, vp]),
     "fode_clamp": (i32, [i32, i32, vp, vp, i32, vp]),
+    "fode_be_residual": (i32, [i32, i32, vp, vp, vp, vp, vp, vp]),
+    "fode_scale_columns": (i32, [i32, i32, vp, i32, vp, vp, vp, vp, vp]),
+    "fode_be_update": (i32, [P(CtrlParams), vp, i32, i32, vp, vp, vp, vp, ct.c_double, vp]),
     "fode_batched_interval": (i32, [vp, vp, P(CtrlParams), P(BatchDesc), i32, vp]),
     "fode_batched_lds_bytes": (i32, [vp, vp, i32, P(ct.c_int64)]),
     "fode_batched_graph_create": (i32, [vp, vp, P(CtrlParams), P(BatchDesc), i32, P(vp)]),

@@ -18,7 +18,7 @@ import torch
 from . import _lib as L
 from .counts import COUNT_FIELDS, Counts, Memory
 from .device import device_mechanism, mechanism_plan
-from .solvers import Algorithm, ctrl_params
+from .solvers import Algorithm, BackwardEuler, RestartingLoop, ctrl_params
 from .system import F64, as_device, device
 from .tableaus import ROS3
 
@@ -53,7 +53,11 @@ class BatchedAlgorithm:
         """stage_tables: copy the plan's tables into LDS when they fit (True),
         never (False), or as the mechanism's tuned choice says (None; stage
         when untuned)."""
-        self.alg = Algorithm(tableau, **options)
+        self._setup(Algorithm(tableau, **options), tableau, group_size, blocks_per_cu, stage_tables,
+                    longest_first)
+
+    def _setup(self, solver, tableau, group_size, blocks_per_cu, stage_tables, longest_first):
+        self.alg = solver
         self.tableau = tableau
         self.group_size = group_size
         self.blocks_per_cu = blocks_per_cu
@@ -81,14 +85,43 @@ class BatchedAlgorithm:
         return (out.cpu().numpy() if not isinstance(Y, torch.Tensor) else out), statuses
 
 
+class BatchedRestartingLoop(BatchedAlgorithm):
+    """``RestartingLoop`` semantics (fastode/solvers.py:189-267) over a whole
+    batch of groups: each call restarts every group from 1e-6 of the
+    interval; no memory is carried (the device blocks keep only Counts)."""
+
+    name = "batched-restarting"
+
+    def __init__(self, tableau=ROS3, group_size=32, blocks_per_cu=0, stage_tables=None, longest_first=True,
+                 max_steps=1000, step_change_rows=True):
+        self._setup(RestartingLoop(tableau, max_steps, step_change_rows), tableau, group_size, blocks_per_cu,
+                    stage_tables, longest_first)
+
+
+class BatchedBackwardEuler(BatchedAlgorithm):
+    """``BackwardEuler`` semantics (fastode/solvers.py:418-491) over a whole
+    batch of groups: every Newton pass of a group runs cell by cell on its
+    persistent workgroup, and the group's convergence decides its step."""
+
+    name = "batched-backward-euler"
+
+    def __init__(self, group_size=32, blocks_per_cu=0, stage_tables=None, longest_first=True,
+                 max_iterations=11, reductions=(0.5, 0.5, 0.5, 0.5, 0.1), small=1.0e-40):
+        self._setup(BackwardEuler(max_iterations, reductions, small), None, group_size, blocks_per_cu,
+                    stage_tables, longest_first)
+
+
 class BatchedRunner:
     """Device buffers and (optionally) the captured interval graph of one
     batch; ``step(kmul)`` advances every group by one interval."""
 
     def __init__(self, balg: BatchedAlgorithm, system, Y, dt, atol, rtol, memory=None, graph=True,
                  record=False):
-        if system.has_constraints and balg.tableau.name != "rodas3":
+        loop = balg.alg.loop
+        if system.has_constraints and loop != L.LOOP_BACKWARD_EULER and balg.tableau.name != "rodas3":
             raise ValueError("constrained systems need RODAS3")
+        if record and loop != L.LOOP_ALGORITHM:
+            raise ValueError("step records are kept for Algorithm's loop only")
         self.balg = balg
         self.system = system
         self.Y = as_device(Y).clone()
@@ -106,7 +139,8 @@ class BatchedRunner:
         self.kmul = torch.ones(system.mech.R, dtype=F64, device=self.Y.device)
         self.atol = as_device(atol)
         # ynew per cell; the projection's corrections and saved state when constrained
-        self.work = torch.empty(((2 if system.has_constraints else 1) * C, n), dtype=F64, device=self.Y.device)
+        projected = system.has_constraints and loop != L.LOOP_BACKWARD_EULER
+        self.work = torch.empty(((2 if projected else 1) * C, n), dtype=F64, device=self.Y.device)
         self.queue = torch.zeros(1, dtype=torch.int32, device=self.Y.device)
         # order the workgroups take groups in: by the previous call's attempts,
         # most first (no effect on results)
@@ -121,7 +155,7 @@ class BatchedRunner:
             g=self.memory.dev.data_ptr(), work=self.work.data_ptr(), queue=self.queue.data_ptr(),
             options=0 if self._stage(balg, system) else 1,
             order=self.order.data_ptr() if balg.longest_first else None)
-        self.hcap = int(balg.alg.max_steps)
+        self.hcap = int(getattr(balg.alg, "max_steps", 0))
         if record:
             # accepted steps of the last call: H per group, and their number
             self.hlog = torch.zeros((self.ngroups, self.hcap), dtype=F64, device=self.Y.device)

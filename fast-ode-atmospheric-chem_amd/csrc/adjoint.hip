@@ -369,6 +369,7 @@ int fode_adjoint_interval(void* plan, void* plan_t, void* mech, const fode_ctrl_
     return afail("fode_adjoint_interval: C, group_size and ngroups disagree");
   if (p->stages < 1 || p->stages > 4) return afail("fode_adjoint_interval: bad tableau");
   if (p->constrained) return afail("fode_adjoint_interval: constrained systems are not supported");
+  if (p->loop != FODE_LOOP_ALGORITHM) return afail("fode_adjoint_interval: only Algorithm's step loop is differentiated");
   if (!a->Y || !a->k || !a->kmul || !a->mass || !a->hlog || !a->nacc || !a->lam || !a->kbar || !a->ws ||
       !a->queue)
     return afail("fode_adjoint_interval: null array");

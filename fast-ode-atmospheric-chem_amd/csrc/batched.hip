@@ -96,6 +96,12 @@ int fode_batched_interval(void* plan, void* mech, const fode_ctrl_params* p, con
     return bfail("fode_batched_interval: null array");
   if ((b->hlog != nullptr) != (b->nacc != nullptr) || (b->hlog && b->hcap < p->max_steps))
     return bfail("fode_batched_interval: hlog and nacc go together, with hcap >= max_steps");
+  if (p->loop < FODE_LOOP_ALGORITHM || p->loop > FODE_LOOP_BACKWARD_EULER)
+    return bfail("fode_batched_interval: unknown step loop");
+  if (p->loop == FODE_LOOP_BACKWARD_EULER && (p->n_reductions < 0 || p->n_reductions > 8 || p->max_iterations < 1))
+    return bfail("fode_batched_interval: bad backward Euler parameters");
+  if (p->loop != FODE_LOOP_ALGORITHM && b->hlog)
+    return bfail("fode_batched_interval: step records are kept for Algorithm's loop only");
   BatchArgs A;
   A.P = *static_cast<const PlanDev*>(fode_internal_plan(plan));
   A.M = *static_cast<const MechDev*>(fode_internal_mech(mech));

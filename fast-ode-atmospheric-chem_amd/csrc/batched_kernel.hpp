@@ -56,6 +56,7 @@ __device__ __forceinline__ double cell_attempt(const BatchArgs& A, const Tables&
     for (int j = tid; j < M.R; j += NT) V.kc[j] = kg[j] * A.b.kmul[j];
   __syncthreads();
   cell_stages<TILE>(A.P, T, M, A.p, A.b.mass, L, V, g.H, g.alpha, ynew_out);
+  if (step_change_rows(A.p)) return cell_err_sum_dae(V.y, V.w, V.fr, A.b.atol, A.b.rtol, n, V.red, A.b.mass);
   return cell_err_sum(V.y, V.w, V.fr, A.b.atol, A.b.rtol, n, V.red);
 }
 
@@ -333,7 +334,7 @@ __global__ __launch_bounds__(NT) void k_batched(BatchArgs A) {
       double* Y = B.Y + (size_t)c0 * n;
       for (long i = tid; i < N; i += NT) {
         if (A.p.constrained && B.mass[i % n] != 1.0) continue;
-        Y[i] = fmax(Y[i], 0.0);
+        Y[i] = np_max0(Y[i]);
       }
     }
     if (tid == 0) {
@@ -348,9 +349,148 @@ __global__ __launch_bounds__(NT) void k_batched(BatchArgs A) {
   }
 }
 
+// np.maximum(a, b): NaN from either side propagates.
+__device__ inline double np_max2(double a, double b) {
+  if (isnan(a)) return a;
+  if (isnan(b)) return b;
+  return a >= b ? a : b;
+}
+
+// One Newton pass of backward Euler on one cell (fastode/solvers.py:450-461):
+// F and J at the iterate (from work), the factorization of I/H - J, the
+// correction for F - (Yn1 - Yn)/H, and the new iterate max(Yn1 + delta, 0)
+// back to work.  Returns (in thread 0's view after the block reductions) a
+// miss flag: some correction non-finite or above its bound
+// (BackwardEuler.converged, :433-438), and sets *nonfinite for an iterate
+// that is not finite.
+template <int TILE>
+__device__ __forceinline__ void be_cell_pass(const BatchArgs& A, const Tables& T, const EngineLds& L,
+                                             const CellLds& V, const fode_group_ctrl& g, int cell,
+                                             double& miss, double& nonfinite) {
+  const MechDev& M = A.M;
+  const PlanDev& P = A.P;
+  const fode_batch_desc& B = A.b;
+  const int n = M.n, tid = threadIdx.x;
+  double* yn1 = B.work + (size_t)cell * n;
+  const double* yn = B.Y + (size_t)cell * n;
+  const double* kg = B.k + (size_t)cell * B.k_stride;
+  for (int i = tid; i < n; i += NT) {
+    V.y[i] = yn1[i];
+    V.w[i] = yn[i];
+  }
+  if (B.k_stride != 0)
+    for (int j = tid; j < M.R; j += NT) V.kc[j] = kg[j] * B.kmul[j];
+  __syncthreads();
+  rates_into(M, V.y, V.kc, V.rd);
+  forcing_into(M, V.y, V.rd, V.f0);
+  jacobian_into(M, V);
+  double a[TILE][TILE];
+  assemble<TILE>(P, JLds{V.jv}, g.alpha, B.mass, false, L.lds, a);
+  factor_cell<TILE>(P, T, L, a);
+  double* vec = L.lds + P.vec;
+  const double H = g.H;
+  for (int q = tid; q < P.c + P.mpad; q += NT) {
+    double v = 0.0;
+    if (q < n) {
+      const int i = P.perm[q];
+      const double d = V.y[i] - V.w[i];
+      v = V.f0[i] - d / H;
+    }
+    vec[q] = v;
+  }
+  solve_cell<TILE>(P, T, L, a);
+  for (int q = tid; q < n; q += NT) V.fr[P.perm[q]] = vec[q];
+  __syncthreads();
+  for (int i = tid; i < n; i += NT) {
+    const double d = V.fr[i];
+    const double v = np_max0(V.y[i] + d);
+    yn1[i] = v;
+    const double bound = np_max2(np_max2(A.p.small, B.atol[i]), B.rtol * fabs(v));
+    if (!isfinite(d) || !(fabs(d) <= bound)) miss = 1.0;
+    if (!isfinite(v)) nonfinite = 1.0;
+  }
+  __syncthreads();
+}
+
+// Batched backward Euler (BackwardEuler.solve, fastode/solvers.py:440-491)
+// on the persistent workgroups: per group, the step's start Yn in Y and the
+// Newton iterate Yn1 in work; each pass runs cell by cell and the group's
+// decision (be_after) follows the block reductions of the flags.
+template <int TILE, bool STAGED>
+__global__ __launch_bounds__(NT) void k_batched_be(BatchArgs A) {
+  extern __shared__ double smem[];
+  const int n = A.M.n, S = A.p.stages;
+  Tables T;
+  if constexpr (STAGED) {
+    T = stage_tables(A.P, reinterpret_cast<uint32_t*>(smem));
+  } else {
+    T = A.P.t;
+  }
+  const EngineLds L = carve(smem + A.lay.engine, A.P);
+  const CellLds V = carve_cell(smem, A.lay, n, A.M.R, S);
+  __shared__ fode_group_ctrl gs;
+  const int tid = threadIdx.x;
+  const fode_batch_desc& B = A.b;
+  if (B.k_stride == 0)
+    for (int j = tid; j < A.M.R; j += NT) V.kc[j] = B.k[j] * B.kmul[j];
+  for (;;) {
+    if (tid == 0) {
+      const int idx = atomicAdd(B.queue, 1);
+      int grp = idx;
+      if (idx < B.ngroups && B.order) {
+        grp = B.order[idx];
+        if (grp < 0 || grp >= B.ngroups) grp = idx;
+      }
+      V.flags[0] = grp;
+    }
+    __syncthreads();
+    const int group = V.flags[0];
+    __syncthreads();
+    if (group >= B.ngroups) break;
+    const int c0 = group * B.group_size;
+    const int cells = min(B.group_size, B.C - c0);
+    const long N = (long)cells * n;
+    double* Y = B.Y + (size_t)c0 * n;
+    double* W = B.work + (size_t)c0 * n;
+    if (tid == 0) {
+      gs = B.g[group];
+      gs.dt = B.dt;
+      ctrl_init(A.p, gs);
+    }
+    for (long i = tid; i < N; i += NT) W[i] = Y[i];
+    __syncthreads();
+    while (gs.action != FODE_FINISH) {
+      double miss = 0.0, nonfinite = 0.0;
+      for (int c = 0; c < cells; ++c) be_cell_pass<TILE>(A, T, L, V, gs, c0 + c, miss, nonfinite);
+      miss = block_max_all(miss, V.red);
+      nonfinite = block_max_all(nonfinite, V.red);
+      if (tid == 0) be_after(A.p, gs, miss == 0.0, nonfinite == 0.0);
+      __syncthreads();
+      if (gs.accepted == FODE_BE_COMMIT) {
+        for (long i = tid; i < N; i += NT) Y[i] = W[i];
+      } else if (gs.accepted == FODE_BE_RESTORE) {
+        for (long i = tid; i < N; i += NT) W[i] = Y[i];
+      }
+      __syncthreads();
+    }
+    // the call returns the clamped iterate (solvers.py:491)
+    for (long i = tid; i < N; i += NT) Y[i] = np_max0(W[i]);
+    if (tid == 0) {
+      if (gs.status != FODE_CONVERGED) {
+        if (gs.fails == 0) gs.first_fail = (int)gs.counts[FODE_C_CALLS] - 1;
+        gs.fails += 1;
+      }
+      B.g[group] = gs;
+      if (B.nacc) B.nacc[group] = 0;
+    }
+    __syncthreads();
+  }
+}
+
 template <int TILE, bool DAE, bool STAGED>
 int launch_batched_as(const BatchArgs& A, int blocks_per_cu, hipStream_t s) {
-  const void* kern = (const void*)k_batched<TILE, DAE, STAGED>;
+  const bool be = A.p.loop == FODE_LOOP_BACKWARD_EULER;
+  const void* kern = be ? (const void*)k_batched_be<TILE, STAGED> : (const void*)k_batched<TILE, DAE, STAGED>;
   const size_t bytes = (size_t)A.lay.total * sizeof(double);
   int dev = 0, cus = 0, maxlds = 0;
   (void)hipGetDevice(&dev);
@@ -368,7 +508,10 @@ int launch_batched_as(const BatchArgs& A, int blocks_per_cu, hipStream_t s) {
   int blocks = cus * per_cu;
   if (blocks > A.b.ngroups) blocks = A.b.ngroups;
   if (blocks < 1) blocks = 1;
-  hipLaunchKernelGGL((k_batched<TILE, DAE, STAGED>), dim3(blocks), dim3(NT), bytes, s, A);
+  if (be)
+    hipLaunchKernelGGL((k_batched_be<TILE, STAGED>), dim3(blocks), dim3(NT), bytes, s, A);
+  else
+    hipLaunchKernelGGL((k_batched<TILE, DAE, STAGED>), dim3(blocks), dim3(NT), bytes, s, A);
   e = hipGetLastError();
   if (e != hipSuccess) return bfail(std::string("k_batched: ") + hipGetErrorString(e));
   return 0;
@@ -377,7 +520,8 @@ int launch_batched_as(const BatchArgs& A, int blocks_per_cu, hipStream_t s) {
 // The persistent kernel for the plan's tile (after the queue reset).
 template <bool STAGED>
 int launch_batched_tiles(const BatchArgs& A, int blocks_per_cu, hipStream_t s) {
-  const bool dae = A.p.constrained != 0;
+  // backward Euler has one kernel for both (it takes no projection)
+  const bool dae = A.p.constrained != 0 && A.p.loop != FODE_LOOP_BACKWARD_EULER;
   switch (A.P.tile) {
 #define X(T)                                                                  \
   case T:                                                                     \

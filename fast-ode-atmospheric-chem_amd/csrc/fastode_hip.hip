@@ -14,6 +14,11 @@
 
 using namespace fode;
 
+// The host binding (the package's _lib.py) lays these structs out with
+// ctypes / NumPy records of exactly these sizes.
+static_assert(sizeof(fode_group_ctrl) == 240, "fode_group_ctrl layout");
+static_assert(sizeof(fode_ctrl_params) == 504, "fode_ctrl_params layout");
+
 namespace {
 
 thread_local std::string g_err;
@@ -205,7 +210,7 @@ __device__ inline void clamp_range(double* Y, long N, int n, const double* mass,
                                    long start, long stride) {
   for (long i = start; i < N; i += stride) {
     if (constrained && mass[i % n] != 1.0) continue;
-    Y[i] = fmax(Y[i], 0.0);
+    Y[i] = np_max0(Y[i]);
   }
 }
 
@@ -234,9 +239,11 @@ __global__ __launch_bounds__(256) void k_norm_control(fode_ctrl_params p, fode_g
   __shared__ int flags[2];
   const long N = (long)C * n;
   double sum = 0.0;
+  const bool step_rows = step_change_rows(p);
   for (int c = 0; c < C; ++c) {
     const size_t o = (size_t)c * n;
-    const double cs = cell_err_sum(Y + o, Ynew + o, E + o, atol, rtol, n, red);
+    const double cs = step_rows ? cell_err_sum_dae(Y + o, Ynew + o, E + o, atol, rtol, n, red, mass)
+                                : cell_err_sum(Y + o, Ynew + o, E + o, atol, rtol, n, red);
     sum = sum + cs;
   }
   if (threadIdx.x == 0) {
@@ -254,6 +261,96 @@ __global__ __launch_bounds__(256) void k_norm_control(fode_ctrl_params p, fode_g
   if (flags[1] == FODE_FINISH) {
     __syncthreads();
     clamp_range(Y, N, n, mass, p.constrained, threadIdx.x, blockDim.x);
+  }
+}
+
+// Backward Euler's Newton right-hand side F - (Yn1 - Yn) / H
+// (fastode/solvers.py:456), H from the group's control block.
+__global__ void k_be_residual(long N, const fode_group_ctrl* __restrict__ g, const double* __restrict__ F,
+                              const double* __restrict__ Yn1, const double* __restrict__ Yn,
+                              double* __restrict__ out) {
+  const double H = g->H;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+    const double d = Yn1[i] - Yn[i];
+    out[i] = F[i] - d / H;
+  }
+}
+
+// np.maximum(a, b) of two doubles: NaN from either side propagates.
+__device__ inline double np_max(double a, double b) {
+  if (isnan(a)) return a;
+  if (isnan(b)) return b;
+  return a >= b ? a : b;
+}
+
+// The rest of a Newton pass over the group (fastode/solvers.py:457-461):
+// Yn1 = max(Yn1 + delta, 0), the convergence test over every entry
+// (BackwardEuler.converged, :433-438), then the step decision (be_after);
+// the iterate becomes the step's start or is reset to it, and at the end of
+// the call it is clamped (:491).  One workgroup.
+__global__ __launch_bounds__(256) void k_be_update(fode_ctrl_params p, fode_group_ctrl* g, int C, int n,
+                                                   double* Yn1, double* Yn, const double* delta,
+                                                   const double* atol, double rtol) {
+  __shared__ double red[4];
+  __shared__ int flags[2];
+  const long N = (long)C * n;
+  double miss = 0.0, nonfinite = 0.0;
+  for (long i = threadIdx.x; i < N; i += blockDim.x) {
+    const double d = delta[i];
+    const double v = np_max0(Yn1[i] + d);
+    Yn1[i] = v;
+    const double bound = np_max(np_max(p.small, atol[i % n]), rtol * fabs(v));
+    if (!isfinite(d) || !(fabs(d) <= bound)) miss = 1.0;
+    if (!isfinite(v)) nonfinite = 1.0;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    miss = fmax(miss, __shfl_xor(miss, off, 64));
+    nonfinite = fmax(nonfinite, __shfl_xor(nonfinite, off, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[threadIdx.x >> 6] = miss;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = 0.0;
+    for (int w = 0; w < 4; ++w) m = fmax(m, red[w]);
+    miss = m;
+  }
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = nonfinite;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = 0.0;
+    for (int w = 0; w < 4; ++w) m = fmax(m, red[w]);
+    fode_group_ctrl c = *g;
+    be_after(p, c, miss == 0.0, m == 0.0);
+    *g = c;
+    flags[0] = c.accepted;
+    flags[1] = c.action;
+  }
+  __syncthreads();
+  if (flags[0] == FODE_BE_COMMIT) {
+    for (long i = threadIdx.x; i < N; i += blockDim.x) Yn[i] = Yn1[i];
+  } else if (flags[0] == FODE_BE_RESTORE) {
+    for (long i = threadIdx.x; i < N; i += blockDim.x) Yn1[i] = Yn[i];
+  }
+  if (flags[1] == FODE_FINISH) {
+    __syncthreads();
+    for (long i = threadIdx.x; i < N; i += blockDim.x) Yn1[i] = np_max0(Yn1[i]);
+  }
+}
+
+// k[c, cols[q]] = (base[c, q] * f) * scale[q]: the photolysis rewrite at an
+// interval start, in the reference's order of products (fastode/ts1.py:205-213).
+__global__ void k_scale_columns(int C, int ldk, double* __restrict__ k, int ncols, const int* __restrict__ cols,
+                                const double* __restrict__ base, const double* __restrict__ scale,
+                                const double* __restrict__ f) {
+  const long total = (long)C * ncols;
+  const double fv = *f;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
+    const int c = (int)(idx / ncols), q = (int)(idx % ncols);
+    const double v = base[idx] * fv;
+    k[(size_t)c * ldk + cols[q]] = v * scale[q];
   }
 }
 
@@ -559,6 +656,38 @@ int fode_norm_control(const fode_ctrl_params* p, fode_group_ctrl* g, int C, int 
   if (!p || !g || C <= 0 || n <= 0) return fail("fode_norm_control: bad argument");
   hipLaunchKernelGGL(k_norm_control, dim3(1), dim3(256), 0, S(stream), *p, g, C, n, Y, Ynew, E, atol,
                      rtol, mass);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+int fode_be_residual(int C, int n, const fode_group_ctrl* g, const double* F, const double* Yn1,
+                     const double* Yn, double* out, void* stream) {
+  if (!g || C < 0 || n <= 0) return fail("fode_be_residual: bad argument");
+  const long N = (long)C * n;
+  if (N == 0) return 0;
+  hipLaunchKernelGGL(k_be_residual, dim3(grid_for(N)), dim3(256), 0, S(stream), N, g, F, Yn1, Yn, out);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+int fode_be_update(const fode_ctrl_params* p, fode_group_ctrl* g, int C, int n, double* Yn1, double* Yn,
+                   const double* delta, const double* atol, double rtol, void* stream) {
+  if (!p || !g || C <= 0 || n <= 0) return fail("fode_be_update: bad argument");
+  if (p->loop != FODE_LOOP_BACKWARD_EULER || p->n_reductions < 0 || p->n_reductions > 8)
+    return fail("fode_be_update: parameters are not backward Euler's");
+  hipLaunchKernelGGL(k_be_update, dim3(1), dim3(256), 0, S(stream), *p, g, C, n, Yn1, Yn, delta, atol, rtol);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+int fode_scale_columns(int C, int ldk, double* k, int ncols, const int32_t* cols, const double* base,
+                       const double* scale, const double* f, void* stream) {
+  if (C < 0 || ncols < 0 || ldk <= 0 || !f) return fail("fode_scale_columns: bad argument");
+  const long total = (long)C * ncols;
+  if (total == 0) return 0;
+  if (!k || !cols || !base || !scale) return fail("fode_scale_columns: null array");
+  hipLaunchKernelGGL(k_scale_columns, dim3(grid_for(total)), dim3(256), 0, S(stream), C, ldk, k, ncols, cols, base,
+                     scale, f);
   HIP_OK(hipGetLastError());
   return 0;
 }

@@ -1,29 +1,36 @@
 """Mass-action mechanisms compiled to the gather tables the HIP kernels read.
 
-A reaction with rate constant k and reactants A_1 ... A_p proceeds at the rate
-r = k x_1 ... x_p; it removes one of each reactant per event and adds the yield
-of each product.  Third bodies (``M``) are not state variables and are folded
-into k by the caller.  This follows the reference's ``Mechanism``
-(fastode/chem.py:19-79): the reactant table ``react`` is (R, P) with the pad
-value n pointing at a column of ones (chem.py:39-41), S is (n, R)
-(chem.py:42-51), and the Jacobian pattern lists the diagonal first, then the
-entries (i, k) in reaction order, reactant order, row order (chem.py:53-77).
+Each reaction j fires at r_j = k_j * (product of its reactant concentrations)
+and changes species i by S[i, j] per event (reactants -1 each, products +
+their yield).  Third-body species (``M``) are not state variables: their
+count per reaction is recorded in ``third_body_power`` and the caller folds
+[M]^power into k.  The arrays are those the reference's ``Mechanism`` exposes
+(fastode/chem.py:19-79): ``react`` (R, P) with the pad value n standing for a
+factor 1, ``S`` (n, R), and the Jacobian pattern ``rows``/``cols`` with the
+diagonal first, then the entries (i, k) met in the order reaction, reactant
+slot, row.
 
-The reference evaluates F = rates @ S^T as a dense GEMM (chem.py:89-90) and the
-Jacobian values as a sparse product GT @ d (chem.py:78-79, 92-102).  Here both
-are compiled to *gather* tables (CSR over the output), so that a device thread
-that owns one output value sums its contributions without atomics:
+The device never forms the dense products the reference evaluates (F =
+rates @ S^T, J = GT @ d, chem.py:86-102).  It reads *gather* tables, CSR over
+the outputs, so that the thread owning one output sums its terms without
+atomics:
 
 * ``rhs_ptr/rhs_rxn/rhs_coef``: for species i, the reactions j with
-  S[i, j] != 0 in increasing j, and S[i, j].
-* ``jac_ptr/jac_src/jac_coef``: for Jacobian entry e = (i, k), the flattened
-  (reaction, reactant slot) indices j*P + p with react[j, p] == k and
-  S[i, j] != 0, in increasing j*P + p, and S[i, j].
+  S[i, j] != 0 in increasing j, and S[i, j];
+* ``jac_ptr/jac_src/jac_coef``: for pattern entry e = (i, k), the flattened
+  slots j*P + p with react[j, p] == k and S[i, j] != 0, increasing, and
+  S[i, j].
 
-Rows listed in ``constraints`` are algebraic linear conservation rows
-g_i(Y) = sum_s c_s Y_s - b_i (as the conservation form of Robertson,
-fastode/problems.py:33-49, replaces one species row by A + B + C - 1); their
-RHS is g_i and their Jacobian row is the constant coefficient vector.
+Linear constraint rows (``constraints``) replace the chemistry of a species'
+row by g_i(Y) = sum_s c_s Y_s - b_i, the conservation form the reference
+writes by hand for Robertson (fastode/problems.py:33-49); their Jacobian row
+is the constant coefficient vector (``jac_const``).
+
+``Mechanism(species, reactions, ...)`` compiles reaction lists;
+``Mechanism.adopt(m)`` compiles any object carrying the reference
+Mechanism's arrays (``species``, ``react``, ``S``, ``pattern.rows/cols``,
+optionally ``third_body_power``), keeping its pattern order, so that a
+mechanism the reference built drives the device systems unchanged.
 """
 from __future__ import annotations
 
@@ -32,124 +39,142 @@ from dataclasses import dataclass, field
 import numpy as np
 
 
+def _csr(keys, n_keys):
+    """Row pointer of sorted integer keys over n_keys rows."""
+    return np.concatenate([[0], np.cumsum(np.bincount(keys, minlength=n_keys))]).astype(np.int64)
+
+
 class Mechanism:
     def __init__(self, species, reactions, third_bodies=("M",), constraints=None):
-        """species: names; reactions: (reactants, products), reactants a list of
-        names (repeats allowed), products a list of (name, yield);
-        constraints: {row species name: ({species name: coefficient}, total)}."""
-        self.species = list(species)
-        n = self.n = len(self.species)
-        index = {s: i for i, s in enumerate(self.species)}
-        R = self.R = len(reactions)
-        lists = []
-        self.third_body_power = np.zeros(R)
-        for j, (reactants, _) in enumerate(reactions):
-            lst = []
-            for s in reactants:
-                if s in third_bodies:
-                    self.third_body_power[j] += 1
-                else:
-                    lst.append(index[s])
-            lists.append(lst)
-        P = self.P = max(1, max((len(lst) for lst in lists), default=1))
-        self.react = np.full((R, P), n, dtype=np.int64)
-        for j, lst in enumerate(lists):
-            self.react[j, : len(lst)] = lst
+        """species: names of the state species; reactions: (reactants,
+        products) pairs, reactants a list of names (repeats allowed),
+        products a list of (name, yield); constraints: {row species name:
+        ({species name: coefficient}, total)}."""
+        species = list(species)
+        index = {name: i for i, name in enumerate(species)}
+        third = frozenset(third_bodies)
+        n, R = len(species), len(reactions)
+        slots = [[index[x] for x in reac if x not in third] for reac, _ in reactions]
+        power = np.array([float(sum(x in third for x in reac)) for reac, _ in reactions])
+        P = max([1] + [len(x) for x in slots])
+        react = np.full((R, P), n, dtype=np.int64)
         S = np.zeros((n, R))
-        for j, (_, products) in enumerate(reactions):
-            for i in lists[j]:
+        for j, ((_, products), x) in enumerate(zip(reactions, slots)):
+            react[j, :len(x)] = x
+            for i in x:
                 S[i, j] -= 1.0
-            for s, y in products:
-                if s in third_bodies:
-                    continue
-                S[index[s], j] += y
-        self.S = S
+            for name, y in products:
+                if name not in third:
+                    S[index[name], j] += y
+        self._compile(species, react, S, power, None, constraints)
 
-        # Linear constraint rows: they replace the chemistry of their row.
-        self.constraint_rows = np.zeros(0, dtype=np.int64)
-        self.constraint_coef = np.zeros((0, n))
-        self.constraint_total = np.zeros(0)
-        if constraints:
-            rows, coefs, totals = [], [], []
-            for name, (coef, total) in constraints.items():
-                rows.append(index[name])
-                c = np.zeros(n)
-                for s, v in coef.items():
-                    c[index[s]] = v
-                coefs.append(c)
-                totals.append(float(total))
-            order = np.argsort(rows)
-            self.constraint_rows = np.asarray(rows, dtype=np.int64)[order]
-            self.constraint_coef = np.asarray(coefs)[order]
-            self.constraint_total = np.asarray(totals)[order]
+    @classmethod
+    def adopt(cls, m, constraints=None):
+        """A device mechanism from any object exposing the reference
+        Mechanism's arrays: ``species``, ``react`` (R, P; pad n), ``S`` (n, R),
+        ``pattern.rows``/``pattern.cols`` (whose order the Jacobian values
+        keep) and, if present, ``third_body_power``."""
+        if isinstance(m, cls):
+            return m
+        self = cls.__new__(cls)
+        species = list(m.species)
+        react = np.asarray(m.react, dtype=np.int64)
+        S = np.asarray(m.S, dtype=np.float64)
+        power = np.asarray(getattr(m, "third_body_power", np.zeros(react.shape[0])), dtype=np.float64)
+        pattern = None
+        if not constraints:
+            pattern = (np.asarray(m.pattern.rows, dtype=np.int64), np.asarray(m.pattern.cols, dtype=np.int64))
+        self._compile(species, react, S, power, pattern, constraints)
+        return self
+
+    # ------------------------------------------------------------------
+    def _compile(self, species, react, S, power, pattern, constraints):
+        n = len(species)
+        R, P = react.shape
+        if S.shape != (n, R):
+            raise ValueError(f"S has shape {S.shape}, expected {(n, R)}")
+        if react.min(initial=0) < 0 or react.max(initial=n) > n:
+            raise ValueError("react holds an index outside 0..n")
+        self.species, self.n, self.R, self.P = species, n, R, P
+        self.react, self.S, self.third_body_power = react, S, power
+        index = {name: i for i, name in enumerate(species)}
+
+        rows_c, coefs, totals = [], [], []
+        for name, (coef, total) in (constraints or {}).items():
+            c = np.zeros(n)
+            for sp_name, v in coef.items():
+                c[index[sp_name]] = v
+            rows_c.append(index[name])
+            coefs.append(c)
+            totals.append(float(total))
+        order = np.argsort(rows_c, kind="stable")
+        self.constraint_rows = np.asarray(rows_c, dtype=np.int64)[order]
+        self.constraint_coef = np.asarray(coefs, dtype=np.float64).reshape(-1, n)[order]
+        self.constraint_total = np.asarray(totals, dtype=np.float64)[order]
         is_con = np.zeros(n, dtype=bool)
         is_con[self.constraint_rows] = True
         self.is_constraint = is_con
 
-        # Jacobian pattern: diagonal first, then reaction order (chem.py:64-77);
-        # constraint rows contribute their coefficient columns afterwards.
-        entries: dict[tuple[int, int], int] = {}
-        rows_l, cols_l = [], []
-
-        def entry(i, k):
-            key = (i, k)
-            if key not in entries:
-                entries[key] = len(rows_l)
-                rows_l.append(i)
-                cols_l.append(k)
-            return entries[key]
-
-        for i in range(n):
-            entry(i, i)
-        contrib: dict[int, list[tuple[int, float]]] = {}
-        for j in range(R):
-            nz = np.nonzero(S[:, j])[0]
+        # chemistry terms of the Jacobian: (row i, column k, slot j*P + p, S[i, j])
+        # in the order reaction j, slot p, row i (the pattern's order)
+        jj, ii = np.nonzero(S.T)
+        t_row, t_col, t_slot = [], [], []
+        by_j = np.split(ii, np.flatnonzero(np.diff(jj)) + 1) if len(jj) else []
+        js = np.unique(jj)
+        for j, rows_j in zip(js, by_j):
+            rows_j = rows_j[~is_con[rows_j]]
             for p in range(P):
-                kk = int(self.react[j, p])
-                if kk == n:
+                k = int(react[j, p])
+                if k == n or not len(rows_j):
                     continue
-                for i in nz:
-                    if is_con[i]:
-                        continue
-                    e = entry(int(i), kk)
-                    contrib.setdefault(e, []).append((j * P + p, float(S[i, j])))
-        con_entries = []
-        for r, row in enumerate(self.constraint_rows):
-            for s in np.nonzero(self.constraint_coef[r])[0]:
-                e = entry(int(row), int(s))
-                con_entries.append((e, float(self.constraint_coef[r, s])))
-        self.rows = np.asarray(rows_l, dtype=np.int64)
-        self.cols = np.asarray(cols_l, dtype=np.int64)
-        self.nnz = len(rows_l)
+                t_row.append(rows_j)
+                t_col.append(np.full(len(rows_j), k))
+                t_slot.append(np.full(len(rows_j), j * P + p))
+        t_row = np.concatenate(t_row) if t_row else np.zeros(0, dtype=np.int64)
+        t_col = np.concatenate(t_col) if t_col else np.zeros(0, dtype=np.int64)
+        t_slot = np.concatenate(t_slot) if t_slot else np.zeros(0, dtype=np.int64)
+        t_coef = S[t_row, t_slot // P]
+
+        con_r, con_c = np.nonzero(self.constraint_coef)
+        con_row = self.constraint_rows[con_r]
+        if pattern is None:
+            # first appearance of each (i, k): diagonal, chemistry terms, constraints
+            cand_r = np.concatenate([np.arange(n), t_row, con_row])
+            cand_c = np.concatenate([np.arange(n), t_col, con_c])
+            key = cand_r * n + cand_c
+            _, first = np.unique(key, return_index=True)
+            first.sort()
+            rows, cols = cand_r[first], cand_c[first]
+        else:
+            rows, cols = pattern
+        self.rows, self.cols = rows.astype(np.int64), cols.astype(np.int64)
+        self.nnz = len(self.rows)
         self.diag = np.arange(n, dtype=np.int64)
+        entry = np.full(n * n, -1, dtype=np.int64)
+        entry[self.rows * n + self.cols] = np.arange(self.nnz)
+        if np.any(entry[np.arange(n) * (n + 1)] < 0):
+            raise ValueError("the pattern lacks a diagonal entry")
 
-        # Gather tables.
-        ptr = [0]
-        rxn, cf = [], []
-        for i in range(n):
-            if not is_con[i]:
-                for j in np.nonzero(S[i])[0]:
-                    rxn.append(int(j))
-                    cf.append(float(S[i, j]))
-            ptr.append(len(rxn))
-        self.rhs_ptr = np.asarray(ptr, dtype=np.int64)
-        self.rhs_rxn = np.asarray(rxn, dtype=np.int64)
-        self.rhs_coef = np.asarray(cf, dtype=np.float64)
+        # species gather: reactions of each row in increasing j
+        si, sj = np.nonzero(S)
+        keep = ~is_con[si]
+        self.rhs_ptr = _csr(si[keep], n)
+        self.rhs_rxn = sj[keep].astype(np.int64)
+        self.rhs_coef = S[si[keep], sj[keep]]
 
-        ptr = [0]
-        src, cf = [], []
-        for e in range(self.nnz):
-            for jp, c in sorted(contrib.get(e, [])):
-                src.append(jp)
-                cf.append(c)
-            ptr.append(len(src))
-        self.jac_ptr = np.asarray(ptr, dtype=np.int64)
-        self.jac_src = np.asarray(src, dtype=np.int64)
-        self.jac_coef = np.asarray(cf, dtype=np.float64)
-        # Constant Jacobian values (constraint rows), added after the gather.
+        # Jacobian gather: terms of each entry in increasing slot
+        t_entry = entry[t_row * n + t_col]
+        if np.any(t_entry < 0):
+            raise ValueError("the pattern lacks entries the mechanism's Jacobian has")
+        o = np.lexsort((t_slot, t_entry))
+        self.jac_ptr = _csr(t_entry[o], self.nnz)
+        self.jac_src = t_slot[o].astype(np.int64)
+        self.jac_coef = t_coef[o]
         self.jac_const = np.zeros(self.nnz)
-        for e, c in con_entries:
-            self.jac_const[e] += c
+        c_entry = entry[con_row * n + con_c]
+        if np.any(c_entry < 0):
+            raise ValueError("the pattern lacks constraint entries")
+        np.add.at(self.jac_const, c_entry, self.constraint_coef[con_r, con_c])
 
     @property
     def has_constraints(self) -> bool:
@@ -162,14 +187,11 @@ class Mechanism:
         out = []
         for j in range(self.R):
             reac = [self.species[k] for k in self.react[j] if k < self.n]
-            prod = []
             col = self.S[:, j].copy()
             for k in self.react[j]:
                 if k < self.n:
                     col[k] += 1.0
-            for i in np.nonzero(col)[0]:
-                prod.append((self.species[i], float(col[i])))
-            out.append((reac, prod))
+            out.append((reac, [(self.species[i], float(col[i])) for i in np.nonzero(col)[0]]))
         return out
 
 

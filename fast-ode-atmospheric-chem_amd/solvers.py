@@ -34,17 +34,35 @@ def quantize(h, interval):
 
 
 def ctrl_params(alg, tab: Tableau, constrained: bool) -> L.CtrlParams:
-    S = tab.stages
+    """The device parameter block of a solver: Algorithm's options, or
+    RestartingLoop's / BackwardEuler's (``alg.loop``)."""
+    loop = getattr(alg, "loop", L.LOOP_ALGORITHM)
     p = L.CtrlParams()
-    p.b, p.safety, p.fac_min, p.fac_max = alg.b, alg.safety, alg.fac_min, alg.fac_max
+    p.loop = loop
+    p.constrained = int(constrained)
+    if loop == L.LOOP_BACKWARD_EULER:
+        if len(alg.reductions) > 8:
+            raise ValueError("backward Euler takes at most 8 reductions")
+        p.stages = 1
+        p.max_iterations = int(alg.max_iterations)
+        p.n_reductions = len(alg.reductions)
+        for i, r in enumerate(alg.reductions):
+            p.reductions[i] = float(r)
+        p.small = float(alg.small)
+        return p
+    S = tab.stages
+    if loop == L.LOOP_RESTARTING:
+        p.step_change_rows = int(alg.step_change_rows)
+        p.max_steps = int(alg.max_steps)
+    else:
+        p.b, p.safety, p.fac_min, p.fac_max = alg.b, alg.safety, alg.fac_min, alg.fac_max
+        p.h211b = int(alg.controller == "h211b")
+        p.carry, p.quantize, p.carry_memory = int(alg.carry), int(alg.quantize), int(alg.carry_memory)
+        p.max_steps = int(alg.max_steps)
     p.k_exp, p.gamma = tab.k, tab.gamma
-    p.h211b = int(alg.controller == "h211b")
-    p.carry, p.quantize, p.carry_memory = int(alg.carry), int(alg.quantize), int(alg.carry_memory)
-    p.max_steps = int(alg.max_steps)
     p.stages = S
     nf = tab.new_forcing
     p.n_new_forcing = int(sum(nf[1:]))
-    p.constrained = int(constrained)
     a = np.zeros((4, 4))
     c = np.zeros((4, 4))
     a[:S, :S] = tab.a
@@ -199,6 +217,7 @@ class Algorithm:
     accepted) tuple per attempt."""
 
     name = "algorithm"
+    loop = L.LOOP_ALGORITHM
 
     def __init__(self, tableau: Tableau = ROS3, controller: str = "h211b", b: float = 4.0,
                  safety: float = 0.9, fac_min: float = 0.2, fac_max: float = 6.0,
@@ -220,54 +239,135 @@ class Algorithm:
         return Memory()
 
     def solve(self, system, Y, dt, atol, rtol, counts: Counts, memory: Memory = None):
-        tab = self.tableau
-        mem = memory if memory is not None else Memory()
-        as_numpy = not isinstance(Y, torch.Tensor)
-        Yd = as_device(Y).clone()
-        C, n = Yd.shape
-        atol_d = as_device(atol)
-        constrained = system.has_constraints
-        params = ctrl_params(self, tab, constrained)
-        ctrl = GroupControl(float(dt), mem)
-        s = L.stream_handle()
-        L.check(L.lib.fode_ctrl_init(ct.byref(params), ctrl.ptr, 1, int(constrained), s), "ctrl_init")
-        host_counts = Counts()
-        if constrained:
-            Yp, st = project(system, Yd, atol, rtol, host_counts)
-            if st != "converged":
-                g = ctrl.read()
-                mem.err_prev, mem.h_prev = float(g["err_prev"]), float(g["h_prev"])
-                counts.add_array(g["counts"])
-                counts.add(host_counts)
-                return self._out(Yp, as_numpy), st
-            Yd = Yp.contiguous()
-            L.check(L.lib.fode_ctrl_begin(ct.byref(params), ctrl.ptr, 1, s), "ctrl_begin")
-        work = {k: torch.empty_like(Yd) for k in ("Yr", "rhs", "Ynew", "E")}
-        K = [torch.empty_like(Yd) for _ in range(tab.stages)]
-        g = ctrl.read()
-        F0 = J = None
-        while g["action"] != L.FINISH:
-            if g["action"] == L.NEW_STEP:
-                F0 = system.rhs(Yd)
-                J = system.jac(Yd)
-            H = float(g["H"])
-            LU = system.factor(ctrl.alpha(), J)
-            Ynew, E = stages(system, tab, params, Yd, F0, LU, ctrl, K, work)
-            L.check(L.lib.fode_norm_control(ct.byref(params), ctrl.ptr, C, n, L.ptr(Yd), L.ptr(Ynew),
-                                            L.ptr(E), L.ptr(atol_d), float(rtol),
-                                            L.ptr(system.mass_dev), s), "fode_norm_control")
+        return rosenbrock_loop(self, system, Y, dt, atol, rtol, counts, memory)
+
+
+def rosenbrock_loop(self, system, Y, dt, atol, rtol, counts: Counts, memory: Memory = None):
+    """The host side of Algorithm.solve and RestartingLoop.solve: one
+    protocol call per operation, the step decisions on the device (the
+    controller named by ``self.loop``)."""
+    tab = self.tableau
+    mem = memory if memory is not None else Memory()
+    as_numpy = not isinstance(Y, torch.Tensor)
+    Yd = as_device(Y).clone()
+    C, n = Yd.shape
+    atol_d = as_device(atol)
+    constrained = system.has_constraints
+    params = ctrl_params(self, tab, constrained)
+    ctrl = GroupControl(float(dt), mem)
+    s = L.stream_handle()
+    L.check(L.lib.fode_ctrl_init(ct.byref(params), ctrl.ptr, 1, int(constrained), s), "ctrl_init")
+    host_counts = Counts()
+    if constrained:
+        Yp, st = project(system, Yd, atol, rtol, host_counts)
+        if st != "converged":
             g = ctrl.read()
-            if self.trace is not None:
-                self.trace.append((H, float(g["err"]), bool(g["accepted"])))
+            mem.err_prev, mem.h_prev = float(g["err_prev"]), float(g["h_prev"])
+            counts.add_array(g["counts"])
+            counts.add(host_counts)
+            return _out(Yp, as_numpy), st
+        Yd = Yp.contiguous()
+        L.check(L.lib.fode_ctrl_begin(ct.byref(params), ctrl.ptr, 1, s), "ctrl_begin")
+    work = {k: torch.empty_like(Yd) for k in ("Yr", "rhs", "Ynew", "E")}
+    K = [torch.empty_like(Yd) for _ in range(tab.stages)]
+    g = ctrl.read()
+    F0 = J = None
+    attempted = False
+    while g["action"] != L.FINISH:
+        attempted = True
+        if g["action"] == L.NEW_STEP:
+            F0 = system.rhs(Yd)
+            J = system.jac(Yd)
+        H = float(g["H"])
+        LU = system.factor(ctrl.alpha(), J)
+        Ynew, E = stages(system, tab, params, Yd, F0, LU, ctrl, K, work)
+        L.check(L.lib.fode_norm_control(ct.byref(params), ctrl.ptr, C, n, L.ptr(Yd), L.ptr(Ynew),
+                                        L.ptr(E), L.ptr(atol_d), float(rtol),
+                                        L.ptr(system.mass_dev), s), "fode_norm_control")
+        g = ctrl.read()
+        if self.trace is not None:
+            self.trace.append((H, float(g["err"]), bool(g["accepted"])))
+    if not attempted:
+        # no attempt ran (a zero budget, an empty interval): the norm/control
+        # kernel that clamps at the end never launched (solvers.py:414)
+        clamp(system, Yd)
+    if self.loop == L.LOOP_ALGORITHM:
         mem.h_next = float(g["h_next"])
         mem.err_prev, mem.h_prev = float(g["err_prev"]), float(g["h_prev"])
-        counts.add_array(g["counts"])
-        counts.add(host_counts)
-        return self._out(Yd, as_numpy), L.STATUS[int(g["status"])]
+    counts.add_array(g["counts"])
+    counts.add(host_counts)
+    return _out(Yd, as_numpy), L.STATUS[int(g["status"])]
 
-    @staticmethod
-    def _out(Y, as_numpy):
-        return Y.cpu().numpy() if as_numpy else Y
+
+def _out(Y, as_numpy):
+    return Y.cpu().numpy() if as_numpy else Y
+
+
+class RestartingLoop:
+    """The baseline Rosenbrock loop (fastode/solvers.py:189-267) on the
+    device: every call starts from 1e-6 of the interval with the elementary
+    controller; no memory is carried.  ``step_change_rows`` replaces the
+    algebraic rows of the error estimate by the step change."""
+
+    name = "restarting"
+    loop = L.LOOP_RESTARTING
+
+    def __init__(self, tableau: Tableau = ROS3, max_steps: int = 1000, step_change_rows: bool = True):
+        self.tableau = tableau
+        self.max_steps = max_steps
+        self.step_change_rows = step_change_rows
+        self.trace = None
+
+    def new_memory(self):
+        return None
+
+    def solve(self, system, Y, dt, atol, rtol, counts: Counts, memory=None):
+        return rosenbrock_loop(self, system, Y, dt, atol, rtol, counts, None)
+
+
+class BackwardEuler:
+    """Backward Euler with Newton's iteration (fastode/solvers.py:418-491):
+    each pass evaluates F and J at the iterate and factors I/H - J through the
+    system, solves for the correction, and the device applies it, tests the
+    group's convergence and takes the step decision (``fode_be_update``)."""
+
+    name = "backward-euler"
+    loop = L.LOOP_BACKWARD_EULER
+
+    def __init__(self, max_iterations: int = 11, reductions=(0.5, 0.5, 0.5, 0.5, 0.1),
+                 small: float = 1.0e-40):
+        self.max_iterations = max_iterations
+        self.reductions = tuple(reductions)
+        self.small = small
+
+    def new_memory(self):
+        return None
+
+    def solve(self, system, Y, dt, atol, rtol, counts: Counts, memory=None):
+        as_numpy = not isinstance(Y, torch.Tensor)
+        Yn = as_device(Y).clone()
+        Yn1 = Yn.clone()
+        C, n = Yn.shape
+        atol_d = as_device(atol)
+        params = ctrl_params(self, None, system.has_constraints)
+        ctrl = GroupControl(float(dt), Memory())
+        s = L.stream_handle()
+        L.check(L.lib.fode_ctrl_init(ct.byref(params), ctrl.ptr, 1, 0, s), "ctrl_init")
+        resid = torch.empty_like(Yn)
+        g = ctrl.read()
+        while g["action"] != L.FINISH:
+            F = system.rhs(Yn1)
+            J = system.jac(Yn1)
+            LU = system.factor(ctrl.alpha(), J)
+            L.check(L.lib.fode_be_residual(C, n, ctrl.ptr, L.ptr(F), L.ptr(Yn1), L.ptr(Yn), L.ptr(resid),
+                                           L.stream_handle()), "fode_be_residual")
+            delta = LU.solve(resid)
+            L.check(L.lib.fode_be_update(ct.byref(params), ctrl.ptr, C, n, L.ptr(Yn1), L.ptr(Yn),
+                                         L.ptr(delta), L.ptr(atol_d), float(rtol), L.stream_handle()),
+                    "fode_be_update")
+            g = ctrl.read()
+        counts.add_array(g["counts"])
+        return _out(Yn1, as_numpy), L.STATUS[int(g["status"])]
 
 
 def fixed_steps(system, tab, Y, t_end, n_steps, counts: Counts = None):

@@ -8,10 +8,17 @@ float64 tensor on the HIP device; ``alpha`` may be a Python float or a
 0-dim device tensor (the device controller hands the latter).
 
 * ``ChemSystem(mech, k, mass)`` mirrors fastode/chem.py:111-134: the forcing,
-  Jacobian values and the factorization run in the HIP kernels.
+  Jacobian values and the factorization run in the HIP kernels.  ``mech`` is
+  a ``fastode_amd.mechanism.Mechanism`` or any object with the reference
+  Mechanism's arrays (compiled once by ``Mechanism.adopt`` and kept on it).
 * ``DenseSystem(n, rhs, jac, mass)`` mirrors fastode/system.py:33-50: rhs and
-  jac are the caller's torch callables, the factorization is the HIP LU on a
-  dense plan.
+  jac are the caller's callables, on torch tensors or on NumPy arrays (found
+  out at the first call), the factorization is the HIP LU on a dense plan.
+
+NumPy arguments get NumPy results: ``rhs(Y)`` and ``LU.solve(B)`` copy a
+host array to the device and the result back, and ``jac(Y)`` returns the
+device values (opaque to the solver, as the protocol has it), so the
+reference's own solvers (fastode/solvers.py) can drive these systems.
 """
 from __future__ import annotations
 
@@ -20,6 +27,7 @@ import torch
 
 from . import _lib as L
 from .device import dense_plan, device_mechanism, mechanism_plan
+from .mechanism import Mechanism
 
 F64 = torch.float64
 
@@ -52,12 +60,28 @@ class HipLU:
         self.n = n
 
     def solve(self, B, out=None):
-        B = B.contiguous()
+        host = not isinstance(B, torch.Tensor)
+        B = as_device(B) if host else B.contiguous()
         C = B.shape[0]
         X = torch.empty_like(B) if out is None else out
         L.check(L.lib.fode_solve(self.plan.handle, C, L.ptr(self.LU), L.ptr(B), L.ptr(X),
                                  L.stream_handle()), "fode_solve")
-        return X
+        return X.cpu().numpy() if host else X
+
+
+def adopted(mech) -> Mechanism:
+    """The device Mechanism of mech: itself, or the compiled form of a
+    reference-built mechanism, made once and kept on the object."""
+    if isinstance(mech, Mechanism):
+        return mech
+    m = getattr(mech, "_fastode_amd_mech", None)
+    if m is None:
+        m = Mechanism.adopt(mech)
+        try:
+            mech._fastode_amd_mech = m
+        except AttributeError:
+            pass
+    return m
 
 
 class _Base:
@@ -89,6 +113,7 @@ class ChemSystem(_Base):
     reference's signature; the device always factors on the symbolic plan."""
 
     def __init__(self, mech, k, mass=None, dense=None):
+        mech = adopted(mech)
         self.mech = mech
         self.n = mech.n
         self.names = mech.species
@@ -115,15 +140,16 @@ class ChemSystem(_Base):
         raise ValueError(f"k has {self.k.shape[0]} rows for {C} cells")
 
     def rhs(self, Y, out=None):
-        Y = Y.contiguous()
+        host = not isinstance(Y, torch.Tensor)
+        Y = as_device(Y) if host else Y.contiguous()
         C = Y.shape[0]
         F = torch.empty_like(Y) if out is None else out
         L.check(L.lib.fode_rhs(self._dm.handle, C, L.ptr(Y), L.ptr(self._k_for(C)), L.ptr(F),
                                L.stream_handle()), "fode_rhs")
-        return F
+        return F.cpu().numpy() if host else F
 
     def jac(self, Y):
-        Y = Y.contiguous()
+        Y = as_device(Y) if not isinstance(Y, torch.Tensor) else Y.contiguous()
         C = Y.shape[0]
         J = torch.empty((C, self.mech.nnz), dtype=F64, device=Y.device)
         L.check(L.lib.fode_jac(self._dm.handle, C, L.ptr(Y), L.ptr(self._k_for(C)), L.ptr(J),
@@ -143,23 +169,44 @@ class ChemSystem(_Base):
 
 
 class DenseSystem(_Base):
-    """A system given by torch callables rhs(Y) -> (C, n) and
-    jac(Y) -> (C, n, n) (fastode/system.py:33-50)."""
+    """A system given by callables rhs(Y) -> (C, n) and jac(Y) -> (C, n, n)
+    (fastode/system.py:33-50), written for torch tensors or for NumPy
+    arrays: a callable that fails on a device tensor is called from then on
+    with host copies, and results come back in the caller's kind of array."""
 
     def __init__(self, n, rhs, jac, mass=None, names=None):
         self.n = n
         self._rhs = rhs
         self._jac = jac
+        self._host = {}
         self.mass_np = np.ones(n) if mass is None else np.asarray(mass, dtype=float)
         self.mass_dev = as_device(self.mass_np)
         self.names = names or [f"y{i}" for i in range(n)]
         self._plan = dense_plan(n)
 
+    def _call(self, which, fn, Y):
+        tensor_in = isinstance(Y, torch.Tensor)
+        if tensor_in and not self._host.get(which, False):
+            try:
+                out = fn(Y)
+            except (TypeError, RuntimeError, AttributeError, ValueError):
+                self._host[which] = True
+                out = fn(Y.cpu().numpy())
+        elif tensor_in:
+            out = fn(Y.cpu().numpy())
+        else:
+            out = fn(Y)
+        if tensor_in and not isinstance(out, torch.Tensor):
+            return as_device(out)
+        if not tensor_in and isinstance(out, torch.Tensor):
+            return out.cpu().numpy()
+        return out
+
     def rhs(self, Y):
-        return self._rhs(Y)
+        return self._call("rhs", self._rhs, Y)
 
     def jac(self, Y):
-        return self._jac(Y)
+        return self._call("jac", self._jac, Y)
 
     def factor(self, alpha, J, algebraic_rows_only=False):
         J = torch.as_tensor(J, dtype=F64, device=device())

@@ -30,11 +30,25 @@ enum {
   FODE_TOO_MANY_STEPS = 2,   /* "too many steps" */
   FODE_STEP_TOO_SMALL = 3,   /* "step too small" */
   FODE_NAN_OR_INF = 4,       /* "nan or inf" */
-  FODE_PROJECTION_FAILED = 5 /* "projection failed" */
+  FODE_PROJECTION_FAILED = 5, /* "projection failed" */
+  FODE_ACCEPTED_UNCONVERGED = 6 /* "accepted unconverged" (backward Euler) */
 };
 
-/* Next action of a group after a control update. */
-enum { FODE_FINISH = 0, FODE_NEW_STEP = 1, FODE_RETRY = 2 };
+/* Next action of a group after a control update: a Rosenbrock attempt with
+ * new F0 and J (NEW_STEP) or with the step's F0 and J (RETRY), a Newton pass
+ * of backward Euler (NEWTON), or none (FINISH). */
+enum { FODE_FINISH = 0, FODE_NEW_STEP = 1, FODE_RETRY = 2, FODE_NEWTON = 3 };
+
+/* The step loop a controller runs (fode_ctrl_params.loop). */
+enum {
+  FODE_LOOP_ALGORITHM = 0,   /* Algorithm.solve (fastode/solvers.py:270-415) */
+  FODE_LOOP_RESTARTING = 1,  /* RestartingLoop.solve (fastode/solvers.py:189-267) */
+  FODE_LOOP_BACKWARD_EULER = 2 /* BackwardEuler.solve (fastode/solvers.py:418-491) */
+};
+
+/* Backward Euler's state after a Newton pass (fode_group_ctrl.accepted):
+ * the iterate becomes the step's start (COMMIT) or is reset to it (RESTORE). */
+enum { FODE_BE_KEEP = 0, FODE_BE_COMMIT = 1, FODE_BE_RESTORE = 2 };
 
 /* Operation counters, 13 x int64 per group, in the order of
  * fastode/counts.py:9-23. */
@@ -46,21 +60,35 @@ enum {
 
 /* Per-group controller state on the device: the local variables of
  * Algorithm.solve (fastode/solvers.py:305-415) plus the carried Memory
- * (fastode/solvers.py:181-186) and the group's Counts. */
+ * (fastode/solvers.py:181-186) and the group's Counts.  RestartingLoop uses
+ * the same fields (no Memory); backward Euler names four of them anew. */
 typedef struct fode_group_ctrl {
   double t, H, H_ctrl, proposed, dt, h_min, h_max, err, alpha;
   double h_next, err_prev, h_prev; /* Memory */
-  int32_t nsteps, status, last, shortened, rej_last, rej_more, action, accepted;
+  int32_t nsteps, status;
+  union { int32_t last; int32_t iterations; };      /* backward Euler: Newton passes of the step */
+  union { int32_t shortened; int32_t converged; };  /* backward Euler: the last pass converged */
+  union { int32_t rej_last; int32_t n_success; };   /* backward Euler: successes in a row */
+  union { int32_t rej_more; int32_t n_fail; };      /* backward Euler: failed steps of the call */
+  int32_t action, accepted;        /* accepted: backward Euler's FODE_BE_* */
   int32_t fails, first_fail;       /* batched form: failed calls, first failing call */
   int64_t counts[FODE_NCOUNTS];
 } fode_group_ctrl;
 
-/* Algorithm options (fastode/solvers.py:275-292) and tableau constants. */
+/* Solver options and tableau constants: Algorithm's options
+ * (fastode/solvers.py:275-292), RestartingLoop's (:194-201), BackwardEuler's
+ * (:423-428); loop selects the step loop (FODE_LOOP_*). */
 typedef struct fode_ctrl_params {
   double b, safety, fac_min, fac_max, k_exp, gamma;
   int32_t h211b, carry, quantize, carry_memory, max_steps, stages, n_new_forcing, constrained;
   double a[16], c[16], m[4], e[4];
   int32_t new_forcing[4];
+  int32_t loop;               /* FODE_LOOP_* */
+  int32_t step_change_rows;   /* RestartingLoop: algebraic rows of E = Ynew - Y */
+  int32_t max_iterations;     /* backward Euler: Newton passes per step */
+  int32_t n_reductions;       /* backward Euler: entries of reductions (<= 8) */
+  double reductions[8];       /* backward Euler: step cuts after failures */
+  double small;               /* backward Euler: least convergence bound */
 } fode_ctrl_params;
 
 /* Mechanism tables (see the package's mechanism.py). */
@@ -153,6 +181,26 @@ int fode_stage_final(const fode_ctrl_params* p, int C, int n, const double* Y,
 int fode_norm_control(const fode_ctrl_params* p, fode_group_ctrl* g, int C, int n, double* Y,
                       const double* Ynew, const double* E, const double* atol, double rtol,
                       const double* mass, void* stream);
+
+/* Backward Euler's Newton pass around the caller's rhs/jac/factor/solve
+ * (BackwardEuler.solve, fastode/solvers.py:448-461): out = F - (Yn1 - Yn)/H
+ * with H from g; then fode_be_update: Yn1 = max(Yn1 + delta, 0), the
+ * group's convergence test (BackwardEuler.converged, :433-438) and the step
+ * decision of :462-486 (the iterate kept, committed to Yn or reset to it),
+ * clamping Yn1 when the call ends (:491).  The set-up of a call is
+ * fode_ctrl_init with p->loop = FODE_LOOP_BACKWARD_EULER. */
+int fode_be_residual(int C, int n, const fode_group_ctrl* g, const double* F, const double* Yn1,
+                     const double* Yn, double* out, void* stream);
+int fode_be_update(const fode_ctrl_params* p, fode_group_ctrl* g, int C, int n, double* Yn1, double* Yn,
+                   const double* delta, const double* atol, double rtol, void* stream);
+
+/* The per-interval photolysis rewrite of run_scenario (fastode/ts1.py:205-213)
+ * on device rate constants: k[c*ldk + cols[q]] = (base[c*ncols + q] * f[0])
+ * * scale[q] for c < C, q < ncols; base holds each cell's photolysis
+ * parameters, scale the reactions' scaling factors, f (a device scalar) the
+ * interval's factor. */
+int fode_scale_columns(int C, int ldk, double* k, int ncols, const int32_t* cols, const double* base,
+                       const double* scale, const double* f, void* stream);
 
 /* clamp(system, Y) alone (fastode/solvers.py:89-97). */
 int fode_clamp(int C, int n, double* Y, const double* mass, int constrained, void* stream);

@@ -174,12 +174,16 @@ class OracleMechanism:
     Optional linear constraint rows replace a species' row by
     sum_s c_s y_s - total (the conservation form of fastode/problems.py:33-49)."""
 
-    def __init__(self, species, reactions, constraints=None):
+    def __init__(self, species, reactions, constraints=None, third_bodies=("M",)):
         self.species = list(species)
         n = self.n = len(species)
         idx = {s: i for i, s in enumerate(species)}
         R = self.R = len(reactions)
-        lists = [[idx[s] for s in reac if s != "M"] for reac, _ in reactions]
+        third = set(third_bodies)
+        # third bodies are not state variables; their count per reaction is
+        # the power of [M] folded into k by the caller (chem.py:28-36)
+        lists = [[idx[s] for s in reac if s not in third] for reac, _ in reactions]
+        self.third_body_power = np.array([float(sum(s in third for s in reac)) for reac, _ in reactions])
         P = self.P = max(1, max((len(x) for x in lists), default=1))
         self.react = np.full((R, P), n, dtype=np.int64)
         for j, x in enumerate(lists):
@@ -189,7 +193,7 @@ class OracleMechanism:
             for i in lists[j]:
                 S[i, j] -= 1.0
             for s, y in prods:
-                if s != "M":
+                if s not in third:
                     S[idx[s], j] += y
         self.con_rows, self.con_coef, self.con_total = [], [], []
         if constraints:
@@ -581,6 +585,170 @@ class Algorithm:
         return clamp(system, Y), status
 
 
+class RestartingLoop:
+    """The baseline Rosenbrock loop (fastode/solvers.py:189-267): every call
+    restarts from 1e-6 of the interval with the elementary controller and
+    fixed constants (safety 0.9, factors in [0.2, 6]); the attempt budget is
+    checked as ``nsteps > max_steps`` at the loop top only; a step is accepted
+    when err < 1 or H < h_min; with constraints the algebraic rows of E are
+    replaced by the step change (step_change_rows)."""
+
+    name = "restarting"
+
+    def __init__(self, tableau=ROS3, max_steps=1000, step_change_rows=True):
+        self.tableau = tableau
+        self.max_steps = max_steps
+        self.step_change_rows = step_change_rows
+        self.trace = None
+
+    def new_memory(self):
+        return None
+
+    def solve(self, system, Y, dt, atol, rtol, counts, memory=None):
+        tab = self.tableau
+        counts.calls += 1
+        h_min, h_max = H_MIN_FRACTION * dt, dt
+        H = min(max(h_min, H_START_FRACTION * dt), h_max)
+        Y = Y.copy()
+        if system.has_constraints:
+            Y, st = project(system, Y, atol, rtol, counts)
+            if st != "converged":
+                return Y, st
+        alg = system.mass == 0.0
+        t = 0.0
+        rej_last = rej_more = False
+        nsteps = 0
+        status = "running"
+        while (t - dt + EPS) <= 0.0 and status == "running":
+            if nsteps > self.max_steps:
+                status = "too many steps"
+                break
+            if (t + 0.1 * H == t) or H <= min(EPS, h_min):
+                status = "step too small"
+                break
+            H = min(H, abs(dt - t))
+            F0 = system.rhs(Y)
+            counts.rhs += 1
+            J = system.jac(Y)
+            counts.jac += 1
+            done = False
+            while not done:
+                LU = system.factor(1.0 / (H * tab.gamma), J)
+                counts.lu += 1
+                Ynew, E = stages(system, tab, Y, F0, LU, H, counts)
+                if system.has_constraints and self.step_change_rows:
+                    E[:, alg] = Ynew[:, alg] - Y[:, alg]
+                err = error_norm(Y, Ynew, E, atol, rtol)
+                nsteps += 1
+                counts.steps += 1
+                if self.trace is not None:
+                    self.trace.append((H, err, err < 1.0 or H < h_min))
+                if math.isnan(err) or math.isinf(err):
+                    status = "nan or inf"
+                    break
+                fac = min(6.0, max(0.2, 0.9 / err ** (1.0 / tab.k)))
+                Hn = H * fac
+                if err < 1.0 or H < h_min:
+                    counts.accepted += 1
+                    t += H
+                    Y = Ynew
+                    Hn = max(h_min, min(Hn, h_max))
+                    if rej_last:
+                        Hn = min(Hn, H)
+                    rej_last = rej_more = False
+                    H = Hn
+                    done = True
+                else:
+                    if rej_more:
+                        Hn = H * 0.1
+                    rej_more = rej_last
+                    rej_last = True
+                    H = Hn
+                    counts.rejected += 1
+        if status == "running":
+            status = "converged"
+        return clamp(system, Y), status
+
+
+class BackwardEuler:
+    """Backward Euler with Newton's iteration (fastode/solvers.py:418-491):
+    each Newton pass evaluates F and J at the iterate, factors I/H - J
+    (mass-weighted) and solves for the correction, then clamps the iterate at
+    0; convergence is tested from the second pass on, over the whole group;
+    a failed step restarts from the step's start with H cut by the next
+    reduction, and after the last reduction the step is taken unconverged;
+    two successes in a row double H."""
+
+    name = "backward-euler"
+
+    def __init__(self, max_iterations=11, reductions=(0.5, 0.5, 0.5, 0.5, 0.1), small=1.0e-40):
+        self.max_iterations = max_iterations
+        self.reductions = tuple(reductions)
+        self.small = small
+
+    def new_memory(self):
+        return None
+
+    def converged(self, delta, Y, atol, rtol):
+        if not np.all(np.isfinite(delta)):
+            return False
+        bound = np.maximum(np.maximum(self.small, atol[None, :]), rtol * np.abs(Y))
+        return bool(np.all(np.abs(delta) <= bound))
+
+    def solve(self, system, Y, dt, atol, rtol, counts, memory=None):
+        counts.calls += 1
+        H = dt
+        t = 0.0
+        n_success = n_fail = 0
+        Yn = Y.copy()
+        Yn1 = Y.copy()
+        status = "converged"
+        while t < dt:
+            ok = False
+            iterations = 0
+            while True:
+                counts.newton += 1
+                F = system.rhs(Yn1)
+                counts.rhs += 1
+                J = system.jac(Yn1)
+                counts.jac += 1
+                LU = system.factor(1.0 / H, J)
+                counts.lu += 1
+                delta = LU.solve(F - (Yn1 - Yn) / H)
+                counts.solves += 1
+                Yn1 = np.maximum(Yn1 + delta, 0.0)
+                iterations += 1
+                if iterations > 1:
+                    ok = self.converged(delta, Yn1, atol, rtol)
+                if ok or iterations >= self.max_iterations:
+                    break
+            counts.steps += 1
+            if not ok:
+                counts.rejected += 1
+                counts.failures += 1
+                n_success = 0
+                if n_fail >= len(self.reductions):
+                    t += H
+                    counts.unconverged += 1
+                    status = "accepted unconverged"
+                    if not np.all(np.isfinite(Yn1)):
+                        status = "nan or inf"
+                    break
+                Yn1 = Yn.copy()
+                H *= self.reductions[n_fail]
+                n_fail += 1
+            else:
+                counts.accepted += 1
+                t += H
+                Yn = Yn1.copy()
+                n_success += 1
+                if n_success >= 2:
+                    n_success = 0
+                    H *= 2.0
+            H = min(H, dt - t)
+        return clamp(system, Yn1), status
+
+
 def fixed_steps(system, tab, Y, t_end, n_steps, counts=None):
     """n uncontrolled equal steps (fastode/solvers.py:494-504)."""
     counts = counts or Counts()
@@ -723,6 +891,11 @@ PINS = {
     "project": "affine covariance; Robertson manifold; golden constrained-config Counts",
     "OracleMechanism": "hand RHS of Robertson; Jacobian vs central differences; golden Y",
     "SuperLU/LapackLU": "residual against numpy.linalg.solve",
+    "RestartingLoop": "exact TS1 Counts of the restart and restart-p rows of the reference's "
+                      "reference-results/ts1/runs.csv; the reference test that a restart costs >= 9 "
+                      "steps where the carried loop takes 1 (tests/test_fastode.py:96-110)",
+    "BackwardEuler": "exact TS1 Counts of the be rows of reference-results/ts1/runs.csv; implicit "
+                     "Euler's closed form on y' = -k y (one Newton pass is exact for a linear system)",
     "tangent_linear": "central differences of Algorithm.solve itself (fixed-step and controlled), "
                       "closed-form dy/dy0 and dy/dlogk of y' = -k y; forcing/jacobian tangents vs "
                       "complex-step derivatives of rhs/jac_values",

@@ -1,0 +1,72 @@
+"""Write the TS1 data fixtures from the reference's own data files (read as
+text; nothing of the reference is imported or run).
+
+* fast-ode-atmospheric-chem_amd/data/ts1.json: the TS1 mechanism
+  configuration (fastode/data/ts1.json), the package's default mechanism.
+* tests/golden/ts1/<scenario>.npz for the eleven scenarios of
+  data/scenarios/: scenario.txt's settings, cells.csv, photolysis.csv,
+  reference_final.csv and, where present, reference_snapshots.csv, each CSV
+  as (header, float64 rows) parsed the way fastode/ts1.py:106-111 parses
+  them (float() of every field).
+* tests/golden/ts1/{runs,be_substeps,rejections}.csv and ts1_families.csv:
+  the reference's recorded results (reference-results/ts1/*.csv,
+  reference-results/dae/ts1_families.csv), copied verbatim.
+
+Run:  python tests/golden/make_ts1_fixtures.py
+"""
+from __future__ import annotations
+
+import csv
+import shutil
+from pathlib import Path
+
+import numpy as np
+
+REF = Path("/root/reference")
+HERE = Path(__file__).resolve().parent
+OUT = HERE / "ts1"
+PKG_DATA = HERE.parents[1] / "fast-ode-atmospheric-chem_amd" / "data"
+
+
+def table(path):
+    with open(path, newline="") as f:
+        rd = csv.reader(f)
+        header = next(rd)
+        rows = [[float(v) for v in row] for row in rd if row]
+    return np.array(header), np.array(rows, dtype=np.float64)
+
+
+def settings(path):
+    out = {}
+    for line in Path(path).read_text().splitlines():
+        line = line.split("#", 1)[0].strip()
+        if line:
+            k, v = (s.strip() for s in line.split("=", 1))
+            out[k] = v
+    return out
+
+
+def main():
+    OUT.mkdir(parents=True, exist_ok=True)
+    PKG_DATA.mkdir(parents=True, exist_ok=True)
+    shutil.copyfile(REF / "fastode/data/ts1.json", PKG_DATA / "ts1.json")
+    for d in sorted((REF / "data/scenarios").iterdir()):
+        if not (d / "scenario.txt").exists():
+            continue
+        s = settings(d / "scenario.txt")
+        arrays = {"dt": np.float64(s["dt"]), "steps": np.int64(s["steps"]),
+                  "snapshot_every": np.int64(s.get("snapshot_every", 0))}
+        arrays["cells_header"], arrays["cells"] = table(d / s["cells"])
+        _, arrays["photolysis"] = table(d / s["photolysis"])
+        arrays["final_header"], arrays["final"] = table(d / "reference_final.csv")
+        if (d / "reference_snapshots.csv").exists():
+            arrays["snap_header"], arrays["snaps"] = table(d / "reference_snapshots.csv")
+        np.savez_compressed(OUT / f"{d.name}.npz", **arrays)
+        print(d.name, {k: getattr(v, "shape", v) for k, v in arrays.items()})
+    for src in ("reference-results/ts1/runs.csv", "reference-results/ts1/be_substeps.csv",
+                "reference-results/ts1/rejections.csv", "reference-results/dae/ts1_families.csv"):
+        shutil.copyfile(REF / src, OUT / Path(src).name)
+
+
+if __name__ == "__main__":
+    main()

@@ -143,4 +143,40 @@ def test_ctrl_record_layout_matches_header():
     import fastode_amd._lib as L
     # 12 doubles, 10 int32, 13 int64 (include/fastode_hip.h: fode_group_ctrl)
     assert L.CTRL_DTYPE.itemsize == 12 * 8 + 10 * 4 + 13 * 8
-    assert ct.sizeof(L.CtrlParams) == 6 * 8 + 8 * 4 + 16 * 8 * 2 + 4 * 8 * 2 + 4 * 4
+    # fode_ctrl_params: 6 doubles, 8 int32, a/c (16 doubles each), m/e (4 each),
+    # new_forcing (4 int32), loop..n_reductions (4 int32), reductions (8), small
+    assert ct.sizeof(L.CtrlParams) == 6 * 8 + 8 * 4 + 16 * 8 * 2 + 4 * 8 * 2 + 4 * 4 + 4 * 4 + 8 * 8 + 8
+
+
+def test_struct_layouts_match_the_c_compiler(tmp_path):
+    """Every field offset of the ctypes / NumPy mirrors equals offsetof() in
+    the header as the C compiler lays it out."""
+    import shutil
+    import subprocess
+    import fastode_amd._lib as L
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    structs = {"fode_ctrl_params": L.CtrlParams, "fode_batch_desc": L.BatchDesc,
+               "fode_mech_desc": L.MechDesc, "fode_plan_desc": L.PlanDesc,
+               "fode_adjoint_desc": L.AdjointDesc}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "fastode_hip.h"', "int main(void) {"]
+    want = []
+    for cname, cls in structs.items():
+        lines.append(f'printf("%zu\\n", sizeof({cname}));')
+        want.append(ct.sizeof(cls))
+        for f, _ in cls._fields_:
+            lines.append(f'printf("%zu\\n", offsetof({cname}, {f}));')
+            want.append(getattr(cls, f).offset)
+    lines.append('printf("%zu\\n", sizeof(fode_group_ctrl));')
+    want.append(L.CTRL_DTYPE.itemsize)
+    for f in L.CTRL_DTYPE.names:
+        lines.append(f'printf("%zu\\n", offsetof(fode_group_ctrl, {f}));')
+        want.append(L.CTRL_DTYPE.fields[f][1])
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-std=c11", "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    assert got == want
