@@ -32,11 +32,17 @@ inline Layout make_layout(const PlanDev& P, const MechDev& M, int S, bool stage)
   l.tables = stage ? (words + 1) / 2 : 0;
   l.engine = l.tables;
   int end = l.engine + engine_lds_doubles(P);
-  // Rates (R) and rate partials (R*P) are needed between the solves, when
-  // only the inverse scratch (scr) is dead; they get a region of their own if
-  // scr is too small.
   const int scr = P.tile * NT;
   const int scr_at = l.engine + engine_scr_offset(P);
+  // Jacobian values are dead once assembled, before the inverse blocks
+  // (dL, dU) are written: they start at dL and may run past the engine
+  // region.
+  l.jval = scr_at + scr;
+  if (l.jval + M.nnz > end) end = l.jval + M.nnz;
+  // Rates (R) and rate partials (R*P) are live together with the Jacobian
+  // values (the partials make them) and between the solves, when only the
+  // inverse scratch (scr) is dead; they get a region of their own, after
+  // everything else, if scr is too small.
   const int need_rd = M.R * M.P > M.R ? M.R * M.P : M.R;
   if (need_rd <= scr) {
     l.rd = scr_at;
@@ -44,10 +50,6 @@ inline Layout make_layout(const PlanDev& P, const MechDev& M, int S, bool stage)
     l.rd = end;
     end += need_rd;
   }
-  // Jacobian values are dead once assembled, before the inverse blocks
-  // (dL, dU) are written.
-  l.jval = scr_at + scr;
-  if (l.jval + M.nnz > end) end = l.jval + M.nnz;
   l.cell = end;
   l.total = l.cell + cell_region_doubles(M.n, M.R, S);
   return l;

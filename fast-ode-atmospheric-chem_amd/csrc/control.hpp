@@ -206,8 +206,10 @@ __device__ inline void be_after(const fode_ctrl_params& p, fode_group_ctrl& g, b
 // np.maximum(x, 0.0): NaN stays NaN (backward Euler's iterate, solvers.py:458).
 __device__ inline double np_max0(double x) { return (x >= 0.0 || isnan(x)) ? x : 0.0; }
 
-// Top of the step loop (solvers.py:330-348).
+// Top of the step loop (solvers.py:330-348).  Backward Euler opens its
+// steps itself (be_init, be_after): nothing to do for it here.
 __device__ inline void ctrl_outer(const fode_ctrl_params& p, fode_group_ctrl& g) {
+  if (p.loop == FODE_LOOP_BACKWARD_EULER) return;
   if (p.loop == FODE_LOOP_RESTARTING) {
     restart_outer(p, g);
     return;
