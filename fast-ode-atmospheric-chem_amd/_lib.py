@@ -128,6 +128,7 @@ _SIGS = {
     "fode_be_update": (i32, [P(CtrlParams), vp, i32, i32, vp, vp, vp, vp, ct.c_double, vp]),
     "fode_batched_interval": (i32, [vp, vp, P(CtrlParams), P(BatchDesc), i32, vp]),
     "fode_batched_lds_bytes": (i32, [vp, vp, i32, P(ct.c_int64)]),
+    "fode_batched_interval_noalloc": (i32, [vp, vp, P(CtrlParams), P(BatchDesc), i32, vp]),
     "fode_batched_graph_create": (i32, [vp, vp, P(CtrlParams), P(BatchDesc), i32, P(vp)]),
     "fode_graph_launch": (i32, [vp, vp]),
     "fode_graph_destroy": (i32, [vp]),

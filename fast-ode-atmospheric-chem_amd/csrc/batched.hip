@@ -13,12 +13,79 @@
 // form forms it, so both forms take the same steps.
 #include "batched_kernel.hpp"
 
+#include <mutex>
+
 using namespace fode;
+
+// Scratch of fode_batched_interval_noalloc: the work queue's counter in its
+// first 256 bytes, then one slot of group_size * n doubles (two for a
+// projected, constrained tableau) per persistent workgroup.  Each GPU the
+// library runs on holds its own copy until the process ends.
+#define FODE_NOALLOC_SCRATCH_BYTES (32u << 20)
+static __device__ __align__(256) double scratch[FODE_NOALLOC_SCRATCH_BYTES / sizeof(double)];
 
 namespace {
 
 __global__ void k_reset_queue(int* q) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *q = 0;
+}
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[MAX_DEVICES] = {};
+
+// The device address of this GPU's scratch, looked up once per device with
+// that device current.  The array's name in host code is only a stand-in for
+// the symbol; kernels get this address.
+int scratch_address(void** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return bfail(std::string("hipGetDevice: ") + hipGetErrorString(e));
+  if (dev < 0 || dev >= MAX_DEVICES) return bfail("fode_batched_interval_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return bfail(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+// Checks shared by the interval entry points; fills A but for work/queue.
+int prepare(void* plan, void* mech, const fode_ctrl_params* p, const fode_batch_desc* b, const char* who,
+            BatchArgs& A) {
+  const std::string w(who);
+  if (!plan || !mech || !p || !b) return bfail(w + ": null argument");
+  if (b->C <= 0 || b->group_size <= 0 || b->ngroups != (b->C + b->group_size - 1) / b->group_size)
+    return bfail(w + ": C, group_size and ngroups disagree");
+  if (p->stages < 1 || p->stages > 4) return bfail(w + ": bad tableau");
+  if (!b->Y || !b->k || !b->kmul || !b->atol || !b->mass || !b->g) return bfail(w + ": null array");
+  if ((b->hlog != nullptr) != (b->nacc != nullptr) || (b->hlog && b->hcap < p->max_steps))
+    return bfail(w + ": hlog and nacc go together, with hcap >= max_steps");
+  if (p->loop < FODE_LOOP_ALGORITHM || p->loop > FODE_LOOP_BACKWARD_EULER) return bfail(w + ": unknown step loop");
+  if (p->loop == FODE_LOOP_BACKWARD_EULER && (p->n_reductions < 0 || p->n_reductions > 8 || p->max_iterations < 1))
+    return bfail(w + ": bad backward Euler parameters");
+  if (p->loop != FODE_LOOP_ALGORITHM && b->hlog) return bfail(w + ": step records are kept for Algorithm's loop only");
+  A.P = *static_cast<const PlanDev*>(fode_internal_plan(plan));
+  A.M = *static_cast<const MechDev*>(fode_internal_mech(mech));
+  if (A.P.n != A.M.n || A.P.nnz != A.M.nnz) return bfail(w + ": plan and mechanism differ");
+  if (A.P.tile < 1 || A.P.tile > 10) return bfail(w + ": unsupported tile");
+  A.p = *p;
+  A.b = *b;
+  A.lay = choose_layout(A.P, A.M, p->stages, b->options);
+  A.work_by_block = 0;
+  A.work_half = 0;
+  A.max_blocks = 0;
+  return 0;
+}
+
+int launch(const BatchArgs& A, int blocks_per_cu, hipStream_t s) {
+  hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, s, A.b.queue);
+  return A.lay.tables > 0 ? launch_batched_tiles<true>(A, blocks_per_cu, s)
+                          : launch_batched_unstaged(A, blocks_per_cu, s);
 }
 
 }  // namespace
@@ -88,32 +155,34 @@ int fode_graph_destroy(void* graph) {
 
 int fode_batched_interval(void* plan, void* mech, const fode_ctrl_params* p, const fode_batch_desc* b,
                           int blocks_per_cu, void* stream) {
-  if (!plan || !mech || !p || !b) return bfail("fode_batched_interval: null argument");
-  if (b->C <= 0 || b->group_size <= 0 || b->ngroups != (b->C + b->group_size - 1) / b->group_size)
-    return bfail("fode_batched_interval: C, group_size and ngroups disagree");
-  if (p->stages < 1 || p->stages > 4) return bfail("fode_batched_interval: bad tableau");
-  if (!b->Y || !b->k || !b->kmul || !b->atol || !b->mass || !b->g || !b->work || !b->queue)
-    return bfail("fode_batched_interval: null array");
-  if ((b->hlog != nullptr) != (b->nacc != nullptr) || (b->hlog && b->hcap < p->max_steps))
-    return bfail("fode_batched_interval: hlog and nacc go together, with hcap >= max_steps");
-  if (p->loop < FODE_LOOP_ALGORITHM || p->loop > FODE_LOOP_BACKWARD_EULER)
-    return bfail("fode_batched_interval: unknown step loop");
-  if (p->loop == FODE_LOOP_BACKWARD_EULER && (p->n_reductions < 0 || p->n_reductions > 8 || p->max_iterations < 1))
-    return bfail("fode_batched_interval: bad backward Euler parameters");
-  if (p->loop != FODE_LOOP_ALGORITHM && b->hlog)
-    return bfail("fode_batched_interval: step records are kept for Algorithm's loop only");
   BatchArgs A;
-  A.P = *static_cast<const PlanDev*>(fode_internal_plan(plan));
-  A.M = *static_cast<const MechDev*>(fode_internal_mech(mech));
-  if (A.P.n != A.M.n || A.P.nnz != A.M.nnz) return bfail("fode_batched_interval: plan and mechanism differ");
-  A.p = *p;
-  A.b = *b;
-  A.lay = choose_layout(A.P, A.M, p->stages, b->options);
-  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-  if (A.P.tile < 1 || A.P.tile > 10) return bfail("fode_batched_interval: unsupported tile");
-  hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, s, A.b.queue);
-  return A.lay.tables > 0 ? launch_batched_tiles<true>(A, blocks_per_cu, s)
-                          : launch_batched_unstaged(A, blocks_per_cu, s);
+  if (prepare(plan, mech, p, b, "fode_batched_interval", A)) return -1;
+  if (!b->work || !b->queue) return bfail("fode_batched_interval: null array");
+  return launch(A, blocks_per_cu, reinterpret_cast<hipStream_t>(stream));
+}
+
+int fode_batched_interval_noalloc(void* plan, void* mech, const fode_ctrl_params* p, const fode_batch_desc* b,
+                                  int blocks_per_cu, void* stream) {
+  BatchArgs A;
+  if (prepare(plan, mech, p, b, "fode_batched_interval_noalloc", A)) return -1;
+  if (b->work || b->queue)
+    return bfail("fode_batched_interval_noalloc: work and queue must be NULL (the library's scratch holds them)");
+  void* base = nullptr;
+  if (scratch_address(&base)) return -1;
+  const bool projected = p->constrained && p->loop != FODE_LOOP_BACKWARD_EULER;
+  const long slot = (long)b->group_size * A.M.n;              // doubles per workgroup and half
+  const long avail = (long)(FODE_NOALLOC_SCRATCH_BYTES / sizeof(double)) - 32;   // after the queue's 256 bytes
+  const long slots = avail / (slot * (projected ? 2 : 1));
+  if (slots < 1)
+    return bfail("fode_batched_interval_noalloc: group_size * n = " + std::to_string(slot) + " doubles" +
+                 (projected ? " (x2 with the projection)" : "") + " exceed the " +
+                 std::to_string(FODE_NOALLOC_SCRATCH_BYTES) + "-byte scratch");
+  A.b.queue = static_cast<int32_t*>(base);
+  A.b.work = static_cast<double*>(base) + 32;
+  A.work_by_block = 1;
+  A.work_half = slots * slot;
+  A.max_blocks = (int)std::min<long>(slots, 1L << 30);
+  return launch(A, blocks_per_cu, reinterpret_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"

@@ -27,7 +27,23 @@ struct BatchArgs {
   fode_ctrl_params p;
   fode_batch_desc b;
   Layout lay;
+  // Where a group's scratch rows live in b.work: at its cells' rows
+  // (work_by_block 0: work is (C, n), its second half from row C), or in the
+  // slot of the workgroup that holds the group (1: group_size * n doubles
+  // per workgroup, the second halves from work_half on).
+  int work_by_block;
+  long work_half;
+  int max_blocks;   // > 0: launch at most this many workgroups (work slots)
 };
+
+// A group's scratch rows (half 0: new states / Newton iterates; half 1: the
+// projection's saved state).
+__device__ __forceinline__ double* group_work(const BatchArgs& A, int c0, int half) {
+  const int n = A.M.n;
+  if (A.work_by_block)
+    return A.b.work + half * A.work_half + (size_t)blockIdx.x * A.b.group_size * n;
+  return A.b.work + ((size_t)half * A.b.C + c0) * n;
+}
 
 // Tables in LDS when everything fits in one workgroup's LDS (and the caller
 // allows it), else in memory.
@@ -124,8 +140,8 @@ __device__ __forceinline__ bool project_group(const BatchArgs& A, const Tables& 
   const int n = A.M.n, tid = threadIdx.x;
   const double* mass = B.mass;
   double* Yg = B.Y + (size_t)c0 * n;
-  double* Dg = B.work + (size_t)c0 * n;
-  double* Og = B.work + ((size_t)B.C + c0) * n;
+  double* Dg = group_work(A, c0, 0);
+  double* Og = group_work(A, c0, 1);
   constexpr int MAX_UPDATES = 10, MAX_BACKTRACKS = 24;
   constexpr double TOL = 0.1, FACTOR = 0.5, SIGMA = 1.0e-4;
   auto count = [&](int which) {
@@ -310,7 +326,7 @@ __global__ __launch_bounds__(NT) void k_batched(BatchArgs A) {
     while (gs.action != FODE_FINISH) {
       double sum = 0.0;
       for (int c = 0; c < cells; ++c) {
-        double* ynew = B.work + (size_t)(c0 + c) * n;
+        double* ynew = group_work(A, c0, 0) + (size_t)c * n;
         const double cs = cell_attempt<TILE>(A, T, L, V, gs, c0 + c, ynew);
         sum = sum + cs;
       }
@@ -323,7 +339,7 @@ __global__ __launch_bounds__(NT) void k_batched(BatchArgs A) {
       if (gs.accepted) {
         const long N = (long)cells * n;
         double* Y = B.Y + (size_t)c0 * n;
-        const double* W = B.work + (size_t)c0 * n;
+        const double* W = group_work(A, c0, 0);
         for (long i = tid; i < N; i += NT) Y[i] = W[i];
       }
       __syncthreads();
@@ -371,7 +387,7 @@ __device__ __forceinline__ void be_cell_pass(const BatchArgs& A, const Tables& T
   const PlanDev& P = A.P;
   const fode_batch_desc& B = A.b;
   const int n = M.n, tid = threadIdx.x;
-  double* yn1 = B.work + (size_t)cell * n;
+  double* yn1 = group_work(A, cell - cell % B.group_size, 0) + (size_t)(cell % B.group_size) * n;
   const double* yn = B.Y + (size_t)cell * n;
   const double* kg = B.k + (size_t)cell * B.k_stride;
   for (int i = tid; i < n; i += NT) {
@@ -451,7 +467,7 @@ __global__ __launch_bounds__(NT) void k_batched_be(BatchArgs A) {
     const int cells = min(B.group_size, B.C - c0);
     const long N = (long)cells * n;
     double* Y = B.Y + (size_t)c0 * n;
-    double* W = B.work + (size_t)c0 * n;
+    double* W = group_work(A, c0, 0);
     if (tid == 0) {
       gs = B.g[group];
       gs.dt = B.dt;
@@ -507,6 +523,7 @@ int launch_batched_as(const BatchArgs& A, int blocks_per_cu, hipStream_t s) {
   const int per_cu = blocks_per_cu > 0 ? blocks_per_cu : occ;
   int blocks = cus * per_cu;
   if (blocks > A.b.ngroups) blocks = A.b.ngroups;
+  if (A.max_blocks > 0 && blocks > A.max_blocks) blocks = A.max_blocks;
   if (blocks < 1) blocks = 1;
   if (be)
     hipLaunchKernelGGL((k_batched_be<TILE, STAGED>), dim3(blocks), dim3(NT), bytes, s, A);

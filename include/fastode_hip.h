@@ -246,6 +246,20 @@ typedef struct fode_batch_desc {
 int fode_batched_interval(void* plan, void* mech, const fode_ctrl_params* p, const fode_batch_desc* b,
                           int blocks_per_cu, void* stream);
 
+/* fode_batched_interval for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  b->work and b->queue must be NULL: the
+ * library's own per-device scratch (32 MiB, a static device array, held by
+ * each GPU until the process ends) takes their place, holding the queue
+ * counter and one slot of group_size * n doubles (two with a projected
+ * constrained tableau) per persistent workgroup; the launch runs at most as
+ * many workgroups as there are slots, and rejects a group_size * n with no
+ * slot at all.  Every other argument, and the results, are those of
+ * fode_batched_interval.  Calls on one GPU share the scratch: calls that
+ * overlap on one GPU (other streams, other threads) race. */
+int fode_batched_interval_noalloc(void* plan, void* mech, const fode_ctrl_params* p, const fode_batch_desc* b,
+                                  int blocks_per_cu, void* stream);
+
 /* LDS bytes one workgroup of the batched kernel uses for this plan/mechanism. */
 int fode_batched_lds_bytes(void* plan, void* mech, int stages, int64_t* bytes);
 
