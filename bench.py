@@ -34,6 +34,8 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "cell-intervals/s, batched vs per-group launch, 200 spp rtol 1e-3; adjoint/fwd"
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+CPU_SHARE = 16     # host processes one GPU's share of the box allows
 V_PASSES = 22      # unfused stage/norm vector passes per attempt counted per species
 
 
@@ -58,21 +60,19 @@ def algorithmic_flops(counts, cells_per_group, n, R, P, nnz_j, fact_flops, nnz_l
     return float(np.sum(per_cell * cells_per_group))
 
 
-def _oracle_job(args):
-    """One group of the CPU sample through the oracle (runs in a worker)."""
-    os.environ.setdefault("OMP_NUM_THREADS", "1")
+def _oracle_setup(name, C, cells=None):
+    """The oracle's mechanism and the workload of a worker: the first C cells
+    (their draws are those of the full workload's first rows), or for the
+    uneven workload the given cells only."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import fastode_oracle as O
     import __graft_entry__
     __graft_entry__.load_package()
     from fastode_amd import workloads as W
-    name, C, group, n_int = args
-    cells = np.arange(group * 32, group * 32 + 32)
     if name == "diurnal":
         wl = W.diurnal200(C=C)
     elif name == "uneven":
         wl = W.uneven200(cells=cells)
-        cells = np.arange(32)
     else:
         wl = W.constrained100(C=C)
     mech = wl.mech
@@ -83,28 +83,66 @@ def _oracle_job(args):
                                   float(mech.constraint_total[i]))
                 for i, r in enumerate(mech.constraint_rows)}
     om = O.OracleMechanism(mech.species, mech.reactions_list(), cons)
+    return O, wl, om
+
+
+_WORKER = {}
+
+
+def _oracle_warm(args):
+    """Build the worker's oracle state (outside the timed region)."""
+    name, C = args
+    if "state" not in _WORKER:
+        _WORKER["state"] = _oracle_setup(name, C, np.arange(32))
+    return os.getpid()
+
+
+def _oracle_job(args):
+    """One group of the CPU sample through the oracle over the GPU's timed
+    intervals; returns (cell-intervals, cell-attempts)."""
+    name, C, group, n_int = args
+    cells = np.arange(group * 32, group * 32 + 32)
+    if name == "uneven":
+        O, wl, om = _oracle_setup(name, C, cells)
+        cells = np.arange(32)
+    else:
+        if "state" not in _WORKER:
+            _WORKER["state"] = _oracle_setup(name, C)
+        O, wl, om = _WORKER["state"]
     alg = O.Algorithm(O.ROS3 if wl.tableau == "ros3" else O.RODAS3)
     mem = alg.new_memory()
     Y = wl.Y0[cells].copy()
+    c = O.Counts()
     for it in range(n_int):
         Y, _ = alg.solve(O.ChemSystem(om, wl.k(it, cells), mass=wl.mass, dense=False), Y, wl.dt, wl.atol,
-                         wl.rtol, O.Counts(), mem)
-    return 32 * n_int
+                         wl.rtol, c, mem)
+    return 32 * n_int, 32 * c.steps
 
 
 def cpu_baseline(name, C, n_groups, n_int):
+    """The oracle on the box's host cores over the same intervals the GPU
+    timed (0 .. n_int-1, from the same initial state), one group per worker
+    process; the pool and the workers' set-up are outside the timed region."""
     import multiprocessing as mp
-    cores = max(1, min(16, len(os.sched_getaffinity(0))))
-    os.environ["OMP_NUM_THREADS"] = "1"
-    os.environ["OPENBLAS_NUM_THREADS"] = "1"
+    affinity = len(os.sched_getaffinity(0))
+    cores = max(1, min(CPU_SHARE, affinity))
+    for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS"):
+        os.environ[k] = "1"
     ctx = mp.get_context("spawn")
-    t0 = time.perf_counter()
     with ctx.Pool(cores) as pool:
-        done = sum(pool.map(_oracle_job, [(name, min(C, 32 * n_groups), g, n_int) for g in range(n_groups)]))
-    wall = time.perf_counter() - t0
+        pool.map(_oracle_warm, [(name, C)] * cores, chunksize=1)
+        t0 = time.perf_counter()
+        res = pool.map(_oracle_job, [(name, C, g, n_int) for g in range(n_groups)], chunksize=1)
+        wall = time.perf_counter() - t0
+    done = sum(r[0] for r in res)
+    attempts = sum(r[1] for r in res)
     return {"value": done / wall, "unit": "cell-intervals/s", "cores": cores, "kind": "port",
-            "sample": f"oracle (NumPy/SciPy SuperLU restatement), first {n_groups} groups x {n_int} "
-                      f"intervals = {done} cell-intervals in {wall:.1f} s, spawn pool, 1 BLAS thread each"}
+            "cell_attempts_per_s": round(attempts / wall, 1),
+            "sample": f"oracle (NumPy/SciPy SuperLU restatement of the reference path), groups 0-{n_groups - 1} "
+                      f"x intervals 0-{n_int - 1} (the GPU's timed intervals, same initial state) = {done} "
+                      f"cell-intervals, {attempts} cell-attempts in {wall:.1f} s; {cores} worker processes "
+                      f"with 1 BLAS thread each ({affinity} cores in the affinity mask, {CPU_SHARE} the "
+                      f"host share of one GPU); pool start-up outside the timing"}
 
 
 def adjoint_leg(fa, torch, W, C, n_int, choice=None):
@@ -156,18 +194,62 @@ def adjoint_leg(fa, torch, W, C, n_int, choice=None):
             "note": "ratio_total = (taped forward + backward) / plain batched forward, same intervals"}
 
 
+def k_resident(wl, torch, dev):
+    """Rate constants resident on the device: one shared row, or per cell."""
+    k = wl.k_base[None, :] if wl.cell_scale is None else wl.k_base[None, :] * wl.cell_scale
+    return torch.as_tensor(k, device=dev)
+
+
+def timed_run(fa, torch, wl, tab, system, K, warmup, blocks_per_cu, longest_first):
+    """Warm a copy of the batch, then time K intervals (graph replays) from
+    the initial state; per-launch times from HIP events on the launch stream."""
+    dev = torch.device("cuda", 0)
+    balg = fa.BatchedAlgorithm(tab, blocks_per_cu=blocks_per_cu, longest_first=longest_first)
+    warm = balg.runner(system, wl.Y0, wl.dt, wl.atol, wl.rtol, graph=True)
+    for it in range(warmup):
+        warm.step(wl.kmul(it))
+    torch.cuda.synchronize()
+    del warm
+    run = balg.runner(system, wl.Y0, wl.dt, wl.atol, wl.rtol, graph=True)
+    stream = torch.cuda.current_stream()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(K)]
+    kms = [torch.as_tensor(wl.kmul(it), dtype=torch.float64, device=dev) for it in range(K)]
+    counts_before = run.memory.counts()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for it in range(K):
+        run.kmul.copy_(kms[it])
+        ev[it][0].record(stream)
+        run.step_graph_only()
+        ev[it][1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    launch_ms = [a.elapsed_time(b) for a, b in ev]
+    rec = run.memory.read()
+    counts = rec["counts"] - counts_before
+    cells_per_group = np.minimum(32, wl.C - 32 * np.arange(run.ngroups))
+    return run, wall, launch_ms, counts, rec, cells_per_group
+
+
+def counter_profile(name):
+    """The committed rocprofv3 counter summary of the kernel on a workload,
+    if there is one (profiles/counters_<workload>.json)."""
+    f = ROOT / "profiles" / f"counters_{name}.json"
+    return json.loads(f.read_text()) if f.exists() else None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=48)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cells", type=int, default=0, help="0: the workload's own size")
     ap.add_argument("--workload", choices=("diurnal", "uneven", "constrained"), default="diurnal",
                     help="diurnal: the headline 200-species workload; uneven: 262,144 cells of uneven "
                          "stiffness; constrained: RODAS3 with conservation rows (100 species)")
     ap.add_argument("--plain-groups", type=int, default=8)
-    ap.add_argument("--cpu-groups", type=int, default=0, help="0: 2 per core")
-    ap.add_argument("--cpu-intervals", type=int, default=1)
+    ap.add_argument("--cpu-groups", type=int, default=0, help="0: one per worker process")
+    ap.add_argument("--cpu-intervals", type=int, default=0, help="0: the GPU's timed intervals")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--blocks-per-cu", type=int, default=0)
@@ -177,6 +259,8 @@ def main():
                     help="first run a plain copy kernel over this many MiB (HBM counter calibration)")
     ap.add_argument("--adjoint-cells", type=int, default=16384, help="0 skips the adjoint leg")
     ap.add_argument("--adjoint-intervals", type=int, default=24)
+    ap.add_argument("--uneven-cells", type=int, default=262144,
+                    help="cells of the uneven-stiffness leg (0 skips it); it runs the timed step count")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: the groups shard "
@@ -201,64 +285,38 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes_read": xs.numel() * 8, "copy_probe_bytes_written": ys.numel() * 8}))
         del xs, ys
+    K = args.steps
     if args.workload == "diurnal":
-        wl = W.diurnal200(C=args.cells or 65536, n_intervals=args.steps)
+        wl = W.diurnal200(C=args.cells or 65536, n_intervals=K)
     elif args.workload == "uneven":
-        wl = W.uneven200(C=args.cells or 262144, n_intervals=args.steps)
+        wl = W.uneven200(C=args.cells or 262144, n_intervals=K)
     else:
-        wl = W.constrained100(C=args.cells or 65536, n_intervals=args.steps)
+        wl = W.constrained100(C=args.cells or 65536, n_intervals=K)
     tab = fa.ROS3 if wl.tableau == "ros3" else fa.RODAS3
     mech = wl.mech
-    K = args.steps
-    # rate constants resident on the device: one shared row, or per cell
-    k_dev = wl.k_base[None, :] if wl.cell_scale is None else wl.k_base[None, :] * wl.cell_scale
-    k_dev = torch.as_tensor(k_dev, device=dev)
-    kmul = wl.kmul
+    k_dev = k_resident(wl, torch, dev)
 
     tuned = None
     if not args.no_tune:
-        # autotuner: one interval of the first 8,192 cells per candidate plan
+        # autotuner: the first 8,192 cells per candidate (a start-up interval,
+        # then a timed steady one)
         from fastode_amd.autotune import tune
         n_s = min(wl.C, 8192)
         t_tune = time.perf_counter()
         ch = tune(mech, k_dev[:n_s] if k_dev.shape[0] > 1 else k_dev, wl.Y0[:n_s], wl.dt, wl.atol, wl.rtol,
-                  tableau=tab, kmul=kmul(0), mass=wl.mass, use_cache=False)
+                  tableau=tab, kmul=wl.kmul(0), mass=wl.mass, use_cache=False)
         tuned = {"rho": ch.rho, "m_max": ch.m_max, "stage_tables": ch.stage_tables, "c": ch.c, "tile": ch.tile,
+                 "blocks_per_cu": ch.blocks_per_cu or "max",
                  "us_per_cell_attempt": round(ch.us_per_cell_attempt, 3),
                  "tuning_s": round(time.perf_counter() - t_tune, 1)}
     system = ChemSystem(mech, k_dev, mass=wl.mass)
-    balg = fa.BatchedAlgorithm(tab, blocks_per_cu=args.blocks_per_cu, longest_first=not args.no_order)
     plan = mechanism_plan(mech).plan
-
-    # warmup on a copy of the batch
-    warm = balg.runner(system, wl.Y0, wl.dt, wl.atol, wl.rtol, graph=True)
-    for it in range(args.warmup):
-        warm.step(kmul(it))
-    torch.cuda.synchronize()
-    del warm
-
-    run = balg.runner(system, wl.Y0, wl.dt, wl.atol, wl.rtol, graph=True)
-    stream = torch.cuda.current_stream()
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(K)]
-    kms = [torch.as_tensor(kmul(it), dtype=torch.float64, device=dev) for it in range(K)]
-    counts_before = run.memory.counts()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for it in range(K):
-        run.kmul.copy_(kms[it])
-        ev[it][0].record(stream)
-        run.step_graph_only()
-        ev[it][1].record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    launch_ms = [a.elapsed_time(b) for a, b in ev]
+    run, wall, launch_ms, counts, rec, cells_per_group = timed_run(
+        fa, torch, wl, tab, system, K, args.warmup, args.blocks_per_cu, not args.no_order)
     kernel_s = sum(launch_ms) / 1e3
-    rec = run.memory.read()
-    counts = rec["counts"] - counts_before
-    cells_per_group = np.minimum(32, wl.C - 32 * np.arange(run.ngroups))
     fails = int((rec["fails"] > 0).sum())
-
     cell_intervals = wl.C * K
+    cell_attempts = float((counts[:, 0] * cells_per_group).sum())
     value = cell_intervals / wall
     fact_flops, nnz_lu = plan.flops()
     fact_flops = 2 * fact_flops
@@ -270,18 +328,29 @@ def main():
     # FETCH_SIZE + WRITE_SIZE of this kernel, scaled by a copy kernel's
     # reading) times this run's cell-attempts per launch
     traffic = None
+    counter_gbs = None
     pmc = ROOT / "profiles" / f"traffic_{wl.name}.json"
     if pmc.exists():
         t = json.loads(pmc.read_text())
         per_attempt = t.get("hbm_bytes_per_cell_attempt")
         if per_attempt:
-            cell_attempts = float((counts[:, 0] * cells_per_group).sum()) / K
-            traffic = round(per_attempt * cell_attempts)
+            traffic = round(per_attempt * cell_attempts / K)
+            counter_gbs = traffic / (kernel_s / K) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "fp64_tflops": round(alg_flops / kernel_s / 1e12, 3), "fp64_frac": round(
-                    alg_flops / kernel_s / 78.6e12, 4),
-                "alg_bytes_per_launch": round(alg_bytes / K), "avg_launch_ms": round(1e3 * kernel_s / K, 3)}
+                "achieved_is": "algorithmic bytes (the compulsory HBM traffic of a streaming, unfused "
+                               "implementation of the same Counts: LU written and read per attempt) per "
+                               "launch / kernel time: a work-rate index, not bytes this kernel moves",
+                "alg_bytes_per_launch": round(alg_bytes / K), "avg_launch_ms": round(1e3 * kernel_s / K, 3),
+                "hbm_counter_gbs": None if counter_gbs is None else round(counter_gbs, 2),
+                "hbm_counter_frac": None if counter_gbs is None else round(counter_gbs / HBM_PEAK_GBS, 5),
+                "fp64_tflops": round(alg_flops / kernel_s / 1e12, 3),
+                "fp64_frac": round(alg_flops / kernel_s / (FP64_PEAK_TFLOPS * 1e12), 4),
+                "us_per_cell_attempt": round(1e6 * kernel_s / cell_attempts, 4)}
+    prof = counter_profile(wl.name)
+    if prof is not None:
+        roofline["measured_binding"] = prof.get("binding")
+        roofline["counters"] = prof.get("summary")
 
     out = {"metric": METRIC, "value": round(value, 1), "unit": "cell-intervals/s", "n_gpus": 1,
            "steps": K, "warmup": args.warmup, "ms_per_step": round(1e3 * wall / K, 3),
@@ -295,6 +364,7 @@ def main():
                       "autotuned": tuned},
            "attempts_per_group_interval": round(float(counts[:, 0].sum()) / (run.ngroups * K), 3),
            "rejected_per_group_interval": round(float(counts[:, 2].sum()) / (run.ngroups * K), 3),
+           "cell_attempts_per_s": round(cell_attempts / wall, 1),
            "failed_groups": fails,
            "roofline": roofline}
 
@@ -324,6 +394,10 @@ def main():
                         "attempts_per_group_interval": round(pc.steps / (G * K), 3),
                         "states_bitwise_equal_to_batched": same}
         out["speedup_vs_plain"] = round(value / plain_value, 1)
+    del run
+
+    if args.uneven_cells > 0 and args.workload == "diurnal":
+        out["uneven"] = uneven_leg(fa, torch, W, args.uneven_cells, K, mech, args)
 
     if args.adjoint_cells > 0:
         out["adjoint"] = adjoint_leg(fa, torch, W, args.adjoint_cells, args.adjoint_intervals,
@@ -331,10 +405,49 @@ def main():
         out["adjoint_over_forward"] = out["adjoint"]["ratio_total"]
 
     if not args.no_cpu:
-        cores = max(1, min(16, len(os.sched_getaffinity(0))))
-        ng = args.cpu_groups or 2 * cores
-        out["cpu_baseline"] = cpu_baseline(args.workload, wl.C, ng, args.cpu_intervals)
+        cores = max(1, min(CPU_SHARE, len(os.sched_getaffinity(0))))
+        ng = args.cpu_groups or cores
+        out["cpu_baseline"] = cpu_baseline(args.workload, 32 * ng, ng, args.cpu_intervals or K)
     print(json.dumps(out))
+
+
+def uneven_leg(fa, torch, W, C, K, tuned_mech, args):
+    """The 262,144-cell workload of uneven stiffness (per-cell temperature
+    and rate multipliers) over the timed step count: throughput, and how
+    unevenly the groups' work falls (attempts per group-interval) against
+    what one persistent workgroup's fair share of a launch is."""
+    from fastode_amd.system import ChemSystem
+    dev = torch.device("cuda", 0)
+    wl = W.uneven200(C=C, n_intervals=K)
+    tab = fa.ROS3
+    choice = getattr(tuned_mech, "_tuned", None)
+    if choice is not None:        # same pattern as the headline mechanism
+        from fastode_amd.autotune import install
+        install(wl.mech, choice)
+    system = ChemSystem(wl.mech, k_resident(wl, torch, dev), mass=wl.mass)
+    run, wall, launch_ms, counts, rec, cpg = timed_run(fa, torch, wl, tab, system, K, 0, args.blocks_per_cu,
+                                                       not args.no_order)
+    per_group = counts[:, 0] / K                      # attempts per group-interval, each group's mean
+    work = counts[:, 0] * cpg                         # cell-attempts per group over the run
+    from fastode_amd import _lib as L
+    cus = L.device_info()["cus"]
+    fair = work.sum() / max(1, min(run.ngroups, cus))   # one workgroup per CU at this LDS size
+    out = {"cells": C, "groups": run.ngroups, "intervals": K, "value": round(C * K / wall, 1),
+           "unit": "cell-intervals/s", "ms_per_interval": round(1e3 * wall / K, 3),
+           "launch_ms_min_median_max": [round(min(launch_ms), 1), round(float(np.median(launch_ms)), 1),
+                                        round(max(launch_ms), 1)],
+           "attempts_per_group_interval": {
+               "mean": round(float(per_group.mean()), 3), "min": round(float(per_group.min()), 3),
+               "median": round(float(np.median(per_group)), 3), "p99": round(float(np.percentile(per_group, 99)), 3),
+               "max": round(float(per_group.max()), 3)},
+           "largest_group_over_fair_share": round(float(work.max() / fair), 4),
+           "cell_attempts_per_s": round(float(work.sum()) / wall, 1),
+           "failed_groups": int((rec["fails"] > 0).sum()),
+           "note": "per-group spread of the work the atomic queue balances (costliest-first order from the "
+                   "previous interval); largest_group_over_fair_share << 1 means no single group can "
+                   "hold up a launch"}
+    del run
+    return out
 
 
 if __name__ == "__main__":

@@ -5,10 +5,14 @@ ordered matrix into sparse pivots (level-scheduled LDS tasks) and the dense
 register-tiled core (the core's size picks the register tile TILE), and on
 whether the plan's tables are staged in LDS.  The tuner builds the distinct
 plans a range of core-density thresholds and core-size caps give, drops the
-ones whose workgroup does not fit in LDS, times one interval of the batched
-kernel on a sample of the caller's cells for each plan, with and without
-table staging, and keeps the fastest per cell-attempt.  Results are cached
-per (pattern, tableau, device) in a JSON file.
+ones whose workgroup does not fit in LDS, and times the batched kernel on a
+sample of the caller's cells for each plan, with and without table staging,
+and at one or at the most workgroups per CU that LDS allows; it keeps the
+fastest per cell-attempt.  Each candidate runs the sample's first interval
+(the start-up transient, from 1e-6 of the interval) untimed and is timed on
+the second, a steady one.  Results are cached per (pattern, tableau, device,
+library build) in a JSON file; a cached choice is re-checked against the
+device's LDS before it is installed, and re-tuned when it no longer fits.
 
 ``tune(...)`` installs the winner as the mechanism's plan, so every later
 solve (plain and batched) uses it.  Every candidate is a valid no-pivot plan
@@ -30,8 +34,21 @@ from .device import DevicePlan
 from .system import F64, ChemSystem
 
 RHOS = (0.6, 0.7, 0.75, 0.8, 0.85, 0.9, 0.95)
-PLAN_FORMAT = 1          # bump when the plan tables or the engine change
 M_MAXES = (160, 144, 128)
+_BUILD = {}
+
+
+def build_id():
+    """A digest of the built library and of the host planner: any change to
+    the kernels, the plan tables or the engine gives a new cache key."""
+    if "id" not in _BUILD:
+        h = hashlib.sha1()
+        h.update(L.LIB_PATH.read_bytes())
+        h.update((Path(__file__).with_name("symbolic.py")).read_bytes())
+        _BUILD["id"] = h.hexdigest()[:12]
+    return _BUILD["id"]
+
+
 CACHE = Path(os.environ.get("FASTODE_AMD_TUNE_CACHE", Path.home() / ".cache" / "fastode_amd" / "tune.json"))
 
 
@@ -43,6 +60,7 @@ class Choice:
     c: int = 0
     tile: int = 0
     us_per_cell_attempt: float = 0.0
+    blocks_per_cu: int = 0       # 0: as many as LDS and registers allow
 
 
 def pattern_key(mech, tableau, device_name):
@@ -50,7 +68,7 @@ def pattern_key(mech, tableau, device_name):
     h.update(np.asarray([mech.n, mech.nnz], dtype=np.int64).tobytes())
     h.update(np.asarray(mech.rows, dtype=np.int64).tobytes())
     h.update(np.asarray(mech.cols, dtype=np.int64).tobytes())
-    return f"{h.hexdigest()[:16]}:{tableau.name}:{device_name}:plan-format-{PLAN_FORMAT}"
+    return f"{h.hexdigest()[:16]}:{tableau.name}:{device_name}:build-{build_id()}"
 
 
 def candidates(mech):
@@ -93,6 +111,18 @@ def install(mech, choice: Choice):
     return mech._plan
 
 
+def fits(mech, choice: Choice, tableau) -> bool:
+    """Whether the choice's workgroup fits the device's LDS (with its tables
+    staged when it asks for that)."""
+    import ctypes as ct
+    from .device import device_mechanism
+    dp = DevicePlan(mech.n, mech.rows, mech.cols, rho=choice.rho, m_max=choice.m_max)
+    b = ct.c_int64()
+    L.check(L.lib.fode_batched_lds_bytes(dp.handle, device_mechanism(mech).handle, tableau.stages,
+                                         ct.byref(b)), "fode_batched_lds_bytes")
+    return b.value + L.CTRL_DTYPE.itemsize <= L.device_info()["lds_per_block"]
+
+
 def tune(mech, k, Y, dt, atol, rtol, tableau=None, kmul=None, mass=None, use_cache=True,
          verbose=False) -> Choice:
     """Time the candidate plans on the sample (k (C, R) or (1, R), Y (C, n):
@@ -105,34 +135,42 @@ def tune(mech, k, Y, dt, atol, rtol, tableau=None, kmul=None, mass=None, use_cac
     key = pattern_key(mech, tableau, dev_name)
     cache = _load_cache() if use_cache else {}
     if key in cache:
-        choice = Choice(**cache[key])
-        install(mech, choice)
-        return choice
+        try:
+            choice = Choice(**cache[key])
+            ok = fits(mech, choice, tableau)
+        except (TypeError, ValueError, L.HipError):
+            ok = False
+        if ok:
+            install(mech, choice)
+            return choice
     saved = getattr(mech, "_plan", None)
     results = []
+    weights = np.minimum(32, len(Y) - 32 * np.arange(-(-len(Y) // 32)))
     for rho, m_max, dp in candidates(mech):
         mech._plan = dp
         for stage in (True, False):
-            balg = BatchedAlgorithm(tableau, stage_tables=stage)
-            try:
-                system = ChemSystem(mech, k, mass=mass)
-                run = balg.runner(system, Y, dt, atol, rtol, graph=False)
-                ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-                torch.cuda.synchronize()
-                ev0.record()
-                run.step(kmul)
-                ev1.record()
-                torch.cuda.synchronize()
-            except L.HipError:
-                continue
-            ms = ev0.elapsed_time(ev1)
-            att = float((run.memory.counts()[:, 0] * np.minimum(
-                balg.group_size, run.C - balg.group_size * np.arange(run.ngroups))).sum())
-            us = 1e3 * ms / max(att, 1.0)
-            results.append(Choice(rho, m_max, stage, dp.plan.c, dp.plan.tile, us))
-            if verbose:
-                print(f"tune: rho {rho} m_max {m_max} c {dp.plan.c} tile {dp.plan.tile} "
-                      f"stage {stage}: {us:.3f} us per cell-attempt")
+            for bpc in (1, 0):
+                balg = BatchedAlgorithm(tableau, stage_tables=stage, blocks_per_cu=bpc)
+                try:
+                    system = ChemSystem(mech, k, mass=mass)
+                    run = balg.runner(system, Y, dt, atol, rtol, graph=False)
+                    run.step(kmul)                  # the start-up interval, untimed
+                    before = run.memory.counts()[:, 0].copy()
+                    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                    torch.cuda.synchronize()
+                    ev0.record()
+                    run.step(kmul)
+                    ev1.record()
+                    torch.cuda.synchronize()
+                except L.HipError:
+                    continue
+                ms = ev0.elapsed_time(ev1)
+                att = float(((run.memory.counts()[:, 0] - before) * weights[:run.ngroups]).sum())
+                us = 1e3 * ms / max(att, 1.0)
+                results.append(Choice(rho, m_max, stage, dp.plan.c, dp.plan.tile, us, bpc))
+                if verbose:
+                    print(f"tune: rho {rho} m_max {m_max} c {dp.plan.c} tile {dp.plan.tile} "
+                          f"stage {stage} blocks/CU {bpc or 'max'}: {us:.3f} us per cell-attempt")
     if not results:
         mech._plan = saved
         raise RuntimeError("no candidate plan fits this device")

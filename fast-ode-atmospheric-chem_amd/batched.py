@@ -166,9 +166,17 @@ class BatchedRunner:
             h = ct.c_void_p()
             L.check(L.lib.fode_batched_graph_create(self.plan.handle, self.dmech.handle,
                                                     ct.byref(self.params), ct.byref(self.desc),
-                                                    int(balg.blocks_per_cu), ct.byref(h)),
+                                                    self._blocks_per_cu(balg, system), ct.byref(h)),
                     "fode_batched_graph_create")
             self.graph = h
+
+    @staticmethod
+    def _blocks_per_cu(balg, system):
+        """The caller's workgroups per CU, else the tuned choice's."""
+        if balg.blocks_per_cu:
+            return int(balg.blocks_per_cu)
+        tuned = getattr(system.mech, "_tuned", None)
+        return int(getattr(tuned, "blocks_per_cu", 0) or 0)
 
     @staticmethod
     def _stage(balg, system):
@@ -212,7 +220,8 @@ class BatchedRunner:
             L.check(L.lib.fode_graph_launch(self.graph, s), "fode_graph_launch")
         else:
             L.check(L.lib.fode_batched_interval(self.plan.handle, self.dmech.handle, ct.byref(self.params),
-                                                ct.byref(self.desc), int(self.balg.blocks_per_cu), s),
+                                                ct.byref(self.desc), self._blocks_per_cu(self.balg, self.system),
+                                                s),
                     "fode_batched_interval")
         if self.balg.longest_first:
             # attempts of this call per group (fode_group_ctrl.nsteps), most first
