@@ -194,17 +194,23 @@ def adjoint_leg(fa, torch, W, C, n_int, choice=None):
             "note": "ratio_total = (taped forward + backward) / plain batched forward, same intervals"}
 
 
+def progress(msg):
+    """A progress line on stderr (stdout carries only the JSON line)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
 def k_resident(wl, torch, dev):
     """Rate constants resident on the device: one shared row, or per cell."""
     k = wl.k_base[None, :] if wl.cell_scale is None else wl.k_base[None, :] * wl.cell_scale
     return torch.as_tensor(k, device=dev)
 
 
-def timed_run(fa, torch, wl, tab, system, K, warmup, blocks_per_cu, longest_first):
+def timed_run(fa, torch, wl, tab, system, K, warmup, blocks_per_cu, longest_first, stage_tables=None):
     """Warm a copy of the batch, then time K intervals (graph replays) from
     the initial state; per-launch times from HIP events on the launch stream."""
     dev = torch.device("cuda", 0)
-    balg = fa.BatchedAlgorithm(tab, blocks_per_cu=blocks_per_cu, longest_first=longest_first)
+    balg = fa.BatchedAlgorithm(tab, blocks_per_cu=blocks_per_cu, longest_first=longest_first,
+                               stage_tables=stage_tables)
     warm = balg.runner(system, wl.Y0, wl.dt, wl.atol, wl.rtol, graph=True)
     for it in range(warmup):
         warm.step(wl.kmul(it))
@@ -253,6 +259,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--blocks-per-cu", type=int, default=0)
+    ap.add_argument("--stage-tables", choices=("auto", "yes", "no"), default="auto",
+                    help="plan tables in LDS (yes), in memory (no), or as tuned (auto)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default plan")
     ap.add_argument("--no-order", action="store_true", help="take groups in index order (not costliest first)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -296,6 +304,7 @@ def main():
     mech = wl.mech
     k_dev = k_resident(wl, torch, dev)
 
+    progress(f"workload {wl.name}: {wl.C} cells, {K} timed intervals")
     tuned = None
     if not args.no_tune:
         # autotuner: the first 8,192 cells per candidate (a start-up interval,
@@ -309,11 +318,14 @@ def main():
                  "blocks_per_cu": ch.blocks_per_cu or "max",
                  "us_per_cell_attempt": round(ch.us_per_cell_attempt, 3),
                  "tuning_s": round(time.perf_counter() - t_tune, 1)}
+        progress(f"tuned: {tuned}")
     system = ChemSystem(mech, k_dev, mass=wl.mass)
     plan = mechanism_plan(mech).plan
+    stage = {"auto": None, "yes": True, "no": False}[args.stage_tables]
     run, wall, launch_ms, counts, rec, cells_per_group = timed_run(
-        fa, torch, wl, tab, system, K, args.warmup, args.blocks_per_cu, not args.no_order)
+        fa, torch, wl, tab, system, K, args.warmup, args.blocks_per_cu, not args.no_order, stage)
     kernel_s = sum(launch_ms) / 1e3
+    progress(f"timed: {wall:.2f} s wall, launches {[round(x, 1) for x in launch_ms]} ms")
     fails = int((rec["fails"] > 0).sum())
     cell_intervals = wl.C * K
     cell_attempts = float((counts[:, 0] * cells_per_group).sum())
@@ -396,10 +408,14 @@ def main():
         out["speedup_vs_plain"] = round(value / plain_value, 1)
     del run
 
+    if not args.no_plain:
+        progress(f"plain path: {out['plain']['value']} cell-intervals/s")
     if args.uneven_cells > 0 and args.workload == "diurnal":
+        progress("uneven leg")
         out["uneven"] = uneven_leg(fa, torch, W, args.uneven_cells, K, mech, args)
 
     if args.adjoint_cells > 0:
+        progress("adjoint leg")
         out["adjoint"] = adjoint_leg(fa, torch, W, args.adjoint_cells, args.adjoint_intervals,
                                      getattr(mech, "_tuned", None))
         out["adjoint_over_forward"] = out["adjoint"]["ratio_total"]
@@ -407,6 +423,7 @@ def main():
     if not args.no_cpu:
         cores = max(1, min(CPU_SHARE, len(os.sched_getaffinity(0))))
         ng = args.cpu_groups or cores
+        progress("CPU baseline")
         out["cpu_baseline"] = cpu_baseline(args.workload, 32 * ng, ng, args.cpu_intervals or K)
     print(json.dumps(out))
 
