@@ -269,6 +269,8 @@ def main():
     ap.add_argument("--adjoint-intervals", type=int, default=24)
     ap.add_argument("--uneven-cells", type=int, default=262144,
                     help="cells of the uneven-stiffness leg (0 skips it); it runs the timed step count")
+    ap.add_argument("--ts1-cells", type=int, default=65536,
+                    help="cells of the TS1 leg (the reference's real mechanism; 0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: the groups shard "
@@ -307,10 +309,10 @@ def main():
     progress(f"workload {wl.name}: {wl.C} cells, {K} timed intervals")
     tuned = None
     if not args.no_tune:
-        # autotuner: the first 8,192 cells per candidate (a start-up interval,
+        # autotuner: the first 16,384 cells per candidate (a start-up interval,
         # then a timed steady one)
         from fastode_amd.autotune import tune
-        n_s = min(wl.C, 8192)
+        n_s = min(wl.C, 16384)
         t_tune = time.perf_counter()
         ch = tune(mech, k_dev[:n_s] if k_dev.shape[0] > 1 else k_dev, wl.Y0[:n_s], wl.dt, wl.atol, wl.rtol,
                   tableau=tab, kmul=wl.kmul(0), mass=wl.mass, use_cache=False)
@@ -414,6 +416,9 @@ def main():
         progress("uneven leg")
         out["uneven"] = uneven_leg(fa, torch, W, args.uneven_cells, K, mech, args)
 
+    if args.ts1_cells > 0 and args.workload == "diurnal":
+        progress("ts1 leg")
+        out["ts1"] = ts1_leg(fa, torch, args.ts1_cells, K, args)
     if args.adjoint_cells > 0:
         progress("adjoint leg")
         out["adjoint"] = adjoint_leg(fa, torch, W, args.adjoint_cells, args.adjoint_intervals,
@@ -426,6 +431,81 @@ def main():
         progress("CPU baseline")
         out["cpu_baseline"] = cpu_baseline(args.workload, 32 * ng, ng, args.cpu_intervals or K)
     print(json.dumps(out))
+
+
+def ts1_leg(fa, torch, C, K, args, scenario="diurnal-10min"):
+    """TS1 (209 species, 547 reactions, the reference's real mechanism) on C
+    cells: the scenario's 64 recorded cells tiled to C, rate constants on the
+    device, the photolysis columns rewritten by fode_scale_columns at every
+    interval start (fastode/ts1.py:205-213), one graph replay per interval;
+    rtol 1e-3, atol 1e-17 (the reference's alg configuration)."""
+    import fastode_amd.ts1 as T
+    from fastode_amd import _lib as L
+    from fastode_amd.autotune import tune
+    from fastode_amd.system import ChemSystem, as_device
+    dev = torch.device("cuda", 0)
+    m = T.TS1()
+    scn = T.Scenario.from_npz(ROOT / "tests" / "golden" / "ts1" / f"{scenario}.npz")
+    base_cells = scn.make_cells(m)
+    reps = -(-C // base_cells.C)
+    idx = np.tile(np.arange(base_cells.C), reps)[:C]
+    T_, P_ = base_cells.T[idx], base_cells.P[idx]
+    params = {k: v[idx] for k, v in base_cells.params.items()}
+    Y0 = base_cells.Y[idx]
+    rtol, atol = 1e-3, np.full(m.mech.n, 1e-17)
+    phot = [k for k in params if k.startswith("j")]
+    f0 = scn.factor(0.0)
+    p0 = dict(params)
+    for name in phot:
+        p0[name] = params[name] * f0
+    k_dev = as_device(m.rate_constants(T_, P_, p0))
+    cols, labels, scale = m.photolysis_table()
+    cols_dev = torch.as_tensor(cols, device=dev)
+    base_dev = as_device(np.stack([params[lab] for lab in labels], axis=1))
+    scale_dev = as_device(scale)
+    f_dev = torch.zeros(1, dtype=torch.float64, device=dev)
+    tuned = None
+    if not args.no_tune:
+        n_s = min(C, 16384)
+        ch = tune(m.mech, k_dev[:n_s], Y0[:n_s], scn.dt, atol, rtol, use_cache=False)
+        tuned = {"c": ch.c, "tile": ch.tile, "stage_tables": ch.stage_tables, "blocks_per_cu": ch.blocks_per_cu or "max",
+                 "us_per_cell_attempt": round(ch.us_per_cell_attempt, 3)}
+        progress(f"ts1 tuned: {tuned}")
+    system = ChemSystem(m.mech, k_dev)
+    balg = fa.BatchedAlgorithm(fa.ROS3, blocks_per_cu=args.blocks_per_cu)
+    run = balg.runner(system, Y0, scn.dt, atol, rtol, graph=True)
+    run.set_kmul(None)
+    stream = torch.cuda.current_stream()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(K)]
+    before = run.memory.counts()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for it in range(K):
+        if it > 0:
+            f_dev.fill_(scn.factor(it * scn.dt))
+            L.check(L.lib.fode_scale_columns(C, k_dev.shape[1], L.ptr(k_dev), len(cols), L.ptr(cols_dev),
+                                             L.ptr(base_dev), L.ptr(scale_dev), L.ptr(f_dev), L.stream_handle()),
+                    "fode_scale_columns")
+        ev[it][0].record(stream)
+        run.step_graph_only()
+        ev[it][1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    launch_ms = [a.elapsed_time(b) for a, b in ev]
+    counts = run.memory.counts() - before
+    cpg = np.minimum(32, C - 32 * np.arange(run.ngroups))
+    attempts = float((counts[:, 0] * cpg).sum())
+    rec = run.memory.read()
+    out = {"scenario": scenario, "cells": C, "species": m.mech.n, "reactions": m.mech.R, "intervals": K,
+           "dt_s": scn.dt, "value": round(C * K / wall, 1), "unit": "cell-intervals/s",
+           "ms_per_interval": round(1e3 * wall / K, 3), "avg_launch_ms": round(sum(launch_ms) / K, 3),
+           "attempts_per_group_interval": round(float(counts[:, 0].sum()) / (run.ngroups * K), 3),
+           "us_per_cell_attempt": round(1e3 * sum(launch_ms) / attempts, 4),
+           "failed_groups": int((rec["fails"] > 0).sum()), "autotuned": tuned,
+           "lu": {"nnz_J": m.mech.nnz, "core": m.mech._plan.plan.m, "tile": m.mech._plan.plan.tile},
+           "note": "the scenario's 64 recorded cells tiled; atol 1e-17, rtol 1e-3"}
+    del run
+    return out
 
 
 def uneven_leg(fa, torch, W, C, K, tuned_mech, args):

@@ -7,8 +7,9 @@ whether the plan's tables are staged in LDS.  The tuner builds the distinct
 plans a range of core-density thresholds and core-size caps give, drops the
 ones whose workgroup does not fit in LDS, and times the batched kernel on a
 sample of the caller's cells for each plan, with and without table staging,
-and at one or at the most workgroups per CU that LDS allows; it keeps the
-fastest per cell-attempt.  Each candidate runs the sample's first interval
+and, for small register tiles with the tables in memory, at one or two
+workgroups (cells) per CU (two need the kernel held to 256 registers per
+lane); it keeps the fastest per cell-attempt.  Each candidate runs the sample's first interval
 (the start-up transient, from 1e-6 of the interval) untimed and is timed on
 the second, a steady one.  Results are cached per (pattern, tableau, device,
 library build) in a JSON file; a cached choice is re-checked against the
@@ -34,6 +35,7 @@ from .device import DevicePlan
 from .system import F64, ChemSystem
 
 RHOS = (0.6, 0.7, 0.75, 0.8, 0.85, 0.9, 0.95)
+OCC2_MAX_TILE = 5        # largest register tile with a two-waves-per-SIMD kernel (tables in memory)
 M_MAXES = (160, 144, 128)
 _BUILD = {}
 
@@ -60,7 +62,7 @@ class Choice:
     c: int = 0
     tile: int = 0
     us_per_cell_attempt: float = 0.0
-    blocks_per_cu: int = 0       # 0: as many as LDS and registers allow
+    blocks_per_cu: int = 1       # workgroups (cells in flight) per CU
 
 
 def pattern_key(mech, tableau, device_name):
@@ -126,8 +128,9 @@ def fits(mech, choice: Choice, tableau) -> bool:
 def tune(mech, k, Y, dt, atol, rtol, tableau=None, kmul=None, mass=None, use_cache=True,
          verbose=False) -> Choice:
     """Time the candidate plans on the sample (k (C, R) or (1, R), Y (C, n):
-    the caller's cells, C about one group per CU or more) over one interval
-    and install the fastest."""
+    the caller's cells, at least two groups per CU for the two-cells-per-CU
+    candidates to show their worth: 16,384 cells on 256 CUs) over one
+    steady interval and install the fastest."""
     from .batched import BatchedAlgorithm
     from .tableaus import ROS3
     tableau = tableau or ROS3
@@ -149,7 +152,7 @@ def tune(mech, k, Y, dt, atol, rtol, tableau=None, kmul=None, mass=None, use_cac
     for rho, m_max, dp in candidates(mech):
         mech._plan = dp
         for stage in (True, False):
-            for bpc in (1, 0):
+            for bpc in ((1, 2) if dp.plan.tile <= OCC2_MAX_TILE and not stage else (1,)):
                 balg = BatchedAlgorithm(tableau, stage_tables=stage, blocks_per_cu=bpc)
                 try:
                     system = ChemSystem(mech, k, mass=mass)

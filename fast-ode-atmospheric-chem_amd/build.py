@@ -9,7 +9,8 @@ import tempfile
 from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
-SRCS = [HERE / "csrc" / f for f in ("fastode_hip.hip", "batched.hip", "batched_unstaged.hip", "adjoint.hip")]
+SRCS = [HERE / "csrc" / f for f in ("fastode_hip.hip", "batched.hip", "batched_unstaged.hip", "batched_occ2.hip",
+                                    "adjoint.hip")]
 LIB = HERE / "libfastode_hip.so"
 ARCH = os.environ.get("FODE_OFFLOAD_ARCH", "gfx950")
 

@@ -82,8 +82,16 @@ int prepare(void* plan, void* mech, const fode_ctrl_params* p, const fode_batch_
   return 0;
 }
 
+// Two or more workgroups per CU asked for, on a small register tile: the
+// instantiation held to 256 registers per lane, with the tables in memory
+// (so that two cells' LDS fits one CU); otherwise one wave per SIMD.
 int launch(const BatchArgs& A, int blocks_per_cu, hipStream_t s) {
   hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, s, A.b.queue);
+  if (blocks_per_cu >= 2 && A.P.tile <= OCC2_MAX_TILE) {
+    BatchArgs B = A;
+    B.lay = make_layout(A.P, A.M, A.p.stages, false);
+    return launch_batched_occ2(B, blocks_per_cu, s);
+  }
   return A.lay.tables > 0 ? launch_batched_tiles<true>(A, blocks_per_cu, s)
                           : launch_batched_unstaged(A, blocks_per_cu, s);
 }

@@ -270,8 +270,11 @@ __device__ __forceinline__ bool project_group(const BatchArgs& A, const Tables& 
 // STAGED: the tables are in LDS (copied at the start); a separate
 // instantiation, so that the compiler addresses them as LDS rather than
 // through generic pointers.
-template <int TILE, bool DAE, bool STAGED>
-__global__ __launch_bounds__(NT) void k_batched(BatchArgs A) {
+// WPE: waves per SIMD the registers are held to (1: as many registers as the
+// code wants, one wave per SIMD; 2: at most 256 per lane, so that two
+// workgroups share a CU when their LDS fits).
+template <int TILE, bool DAE, bool STAGED, int WPE>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void k_batched(BatchArgs A) {
   extern __shared__ double smem[];
   const int n = A.M.n, S = A.p.stages;
   Tables T;
@@ -432,8 +435,8 @@ __device__ __forceinline__ void be_cell_pass(const BatchArgs& A, const Tables& T
 // on the persistent workgroups: per group, the step's start Yn in Y and the
 // Newton iterate Yn1 in work; each pass runs cell by cell and the group's
 // decision (be_after) follows the block reductions of the flags.
-template <int TILE, bool STAGED>
-__global__ __launch_bounds__(NT) void k_batched_be(BatchArgs A) {
+template <int TILE, bool STAGED, int WPE>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void k_batched_be(BatchArgs A) {
   extern __shared__ double smem[];
   const int n = A.M.n, S = A.p.stages;
   Tables T;
@@ -503,10 +506,10 @@ __global__ __launch_bounds__(NT) void k_batched_be(BatchArgs A) {
   }
 }
 
-template <int TILE, bool DAE, bool STAGED>
+template <int TILE, bool DAE, bool STAGED, int WPE>
 int launch_batched_as(const BatchArgs& A, int blocks_per_cu, hipStream_t s) {
   const bool be = A.p.loop == FODE_LOOP_BACKWARD_EULER;
-  const void* kern = be ? (const void*)k_batched_be<TILE, STAGED> : (const void*)k_batched<TILE, DAE, STAGED>;
+  const void* kern = be ? (const void*)k_batched_be<TILE, STAGED, WPE> : (const void*)k_batched<TILE, DAE, STAGED, WPE>;
   const size_t bytes = (size_t)A.lay.total * sizeof(double);
   int dev = 0, cus = 0, maxlds = 0;
   (void)hipGetDevice(&dev);
@@ -520,15 +523,16 @@ int launch_batched_as(const BatchArgs& A, int blocks_per_cu, hipStream_t s) {
   int occ = 0;
   e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, NT, bytes);
   if (e != hipSuccess || occ < 1) occ = 1;
-  const int per_cu = blocks_per_cu > 0 ? blocks_per_cu : occ;
+  // never more persistent workgroups than can be resident at once
+  const int per_cu = blocks_per_cu > 0 && blocks_per_cu < occ ? blocks_per_cu : occ;
   int blocks = cus * per_cu;
   if (blocks > A.b.ngroups) blocks = A.b.ngroups;
   if (A.max_blocks > 0 && blocks > A.max_blocks) blocks = A.max_blocks;
   if (blocks < 1) blocks = 1;
   if (be)
-    hipLaunchKernelGGL((k_batched_be<TILE, STAGED>), dim3(blocks), dim3(NT), bytes, s, A);
+    hipLaunchKernelGGL((k_batched_be<TILE, STAGED, WPE>), dim3(blocks), dim3(NT), bytes, s, A);
   else
-    hipLaunchKernelGGL((k_batched<TILE, DAE, STAGED>), dim3(blocks), dim3(NT), bytes, s, A);
+    hipLaunchKernelGGL((k_batched<TILE, DAE, STAGED, WPE>), dim3(blocks), dim3(NT), bytes, s, A);
   e = hipGetLastError();
   if (e != hipSuccess) return bfail(std::string("k_batched: ") + hipGetErrorString(e));
   return 0;
@@ -542,16 +546,21 @@ int launch_batched_tiles(const BatchArgs& A, int blocks_per_cu, hipStream_t s) {
   switch (A.P.tile) {
 #define X(T)                                                                  \
   case T:                                                                     \
-    return dae ? launch_batched_as<T, true, STAGED>(A, blocks_per_cu, s)      \
-               : launch_batched_as<T, false, STAGED>(A, blocks_per_cu, s);
+    return dae ? launch_batched_as<T, true, STAGED, 1>(A, blocks_per_cu, s)   \
+               : launch_batched_as<T, false, STAGED, 1>(A, blocks_per_cu, s);
     X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10)
 #undef X
   }
   return bfail("fode_batched_interval: unsupported tile");
 }
 
+// Register tiles up to this size have a two-waves-per-SIMD instantiation.
+constexpr int OCC2_MAX_TILE = 5;
+
 // Tables in memory: instantiated in batched_unstaged.hip (its own
-// translation unit, compiled in parallel).
+// translation unit, compiled in parallel); and the two-waves-per-SIMD
+// instantiations (tables in memory) in batched_occ2.hip.
 int launch_batched_unstaged(const BatchArgs& A, int blocks_per_cu, hipStream_t s);
+int launch_batched_occ2(const BatchArgs& A, int blocks_per_cu, hipStream_t s);
 
 }  // namespace fode

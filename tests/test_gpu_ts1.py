@@ -59,15 +59,15 @@ def runs_row(config, rtol, scenario):
     raise KeyError((config, rtol, scenario))
 
 
-def solver_for(kind, options, batched, tableau=None):
+def solver_for(kind, options, batched, tableau=None, blocks_per_cu=0):
     import fastode_amd as fa
     tab = tableau or fa.ROS3
     if batched:
         if kind == "algorithm":
-            return fa.BatchedAlgorithm(tab, **options)
+            return fa.BatchedAlgorithm(tab, blocks_per_cu=blocks_per_cu, **options)
         if kind == "restarting":
-            return fa.BatchedRestartingLoop(tab)
-        return fa.BatchedBackwardEuler()
+            return fa.BatchedRestartingLoop(tab, blocks_per_cu=blocks_per_cu)
+        return fa.BatchedBackwardEuler(blocks_per_cu=blocks_per_cu)
     if kind == "algorithm":
         return fa.Algorithm(tab, **options)
     if kind == "restarting":
@@ -75,14 +75,14 @@ def solver_for(kind, options, batched, tableau=None):
     return fa.BackwardEuler()
 
 
-def run(config, rtol, scenario, batched, cells=64):
+def run(config, rtol, scenario, batched, cells=64, blocks_per_cu=0):
     import fastode_amd.ts1 as T
     kind, options, atol = CONFIGS[config]
     atol = rtol * FLOOR if atol is None else atol
     m = ts1()
     scn = T.Scenario.from_npz(GOLD / f"{scenario}.npz")
     cl = scn.make_cells(m, which=range(cells))
-    solver = solver_for(kind, options, batched)
+    solver = solver_for(kind, options, batched, blocks_per_cu=blocks_per_cu)
     t = time.time()
     if batched:
         snaps, counts, statuses, _ = T.run_scenario_batched(scn, solver, cl, m, rtol, atol)
@@ -145,6 +145,19 @@ def test_batched_ts1_states_equal_the_plain_path(config, rtol):
     b = run(config, rtol, "tier1-30s", batched=True)[0]
     for step in a:
         assert np.array_equal(a[step], b[step]), (config, step, np.abs(a[step] - b[step]).max())
+
+
+@pytest.mark.parametrize("config,rtol", [("alg", 1e-3), ("restart-p", 1e-3), ("be", 1e-6)])
+def test_batched_ts1_two_cells_per_cu(config, rtol):
+    """The kernel held to two waves per SIMD, two workgroups (cells) per CU,
+    tables in memory: the recorded Counts, and the one-wave kernel's states
+    bit for bit (the same arithmetic, other registers)."""
+    a = run(config, rtol, "tier1-30s", batched=True, blocks_per_cu=2)
+    b = run(config, rtol, "tier1-30s", batched=True)
+    ref = runs_row(config, rtol, "tier1-30s")
+    assert {c: getattr(a[1], c) for c in COUNT_COLS} == {c: int(ref[c]) for c in COUNT_COLS}
+    for step in a[0]:
+        assert np.array_equal(a[0][step], b[0][step]), (config, step)
 
 
 def family_row(rtol, form, tableau, solver):
