@@ -205,7 +205,20 @@ def k_resident(wl, torch, dev):
     return torch.as_tensor(k, device=dev)
 
 
-def timed_run(fa, torch, wl, tab, system, K, warmup, blocks_per_cu, longest_first, stage_tables=None):
+def read_phase_timers(reset=False):
+    """Cycles per phase of the timed library's batched kernel (summed over
+    workgroups since the last reset)."""
+    import ctypes as ct
+    from fastode_amd import _lib as L
+    buf = (ct.c_uint64 * len(L.PHASES))()
+    n = L.lib.fode_phase_timers(buf, len(L.PHASES), int(reset))
+    if n < 0:
+        raise RuntimeError(L.lib.fode_last_error().decode())
+    return dict(zip(L.PHASES, (int(x) for x in buf)))
+
+
+def timed_run(fa, torch, wl, tab, system, K, warmup, blocks_per_cu, longest_first, stage_tables=None,
+              phase_timers=False):
     """Warm a copy of the batch, then time K intervals (graph replays) from
     the initial state; per-launch times from HIP events on the launch stream."""
     dev = torch.device("cuda", 0)
@@ -222,6 +235,8 @@ def timed_run(fa, torch, wl, tab, system, K, warmup, blocks_per_cu, longest_firs
     kms = [torch.as_tensor(wl.kmul(it), dtype=torch.float64, device=dev) for it in range(K)]
     counts_before = run.memory.counts()
     torch.cuda.synchronize()
+    if phase_timers:
+        read_phase_timers(reset=True)
     t0 = time.perf_counter()
     for it in range(K):
         run.kmul.copy_(kms[it])
@@ -234,6 +249,15 @@ def timed_run(fa, torch, wl, tab, system, K, warmup, blocks_per_cu, longest_firs
     rec = run.memory.read()
     counts = rec["counts"] - counts_before
     cells_per_group = np.minimum(32, wl.C - 32 * np.arange(run.ngroups))
+    if phase_timers:
+        cyc = read_phase_timers()
+        attempts = float((counts[:, 0] * cells_per_group).sum())
+        total = sum(cyc.values())
+        run.phase_report = {"cycles_per_cell_attempt": round(total / attempts),
+                            "share": {k: round(v / total, 4) for k, v in cyc.items()},
+                            "cycles_per_cell_attempt_by_phase": {k: round(v / attempts) for k, v in cyc.items()},
+                            "note": "workgroup cycles (shader clock, thread 0's marks) summed over workgroups, "
+                                    "over the timed intervals, per cell-attempt"}
     return run, wall, launch_ms, counts, rec, cells_per_group
 
 
@@ -269,6 +293,9 @@ def main():
     ap.add_argument("--adjoint-intervals", type=int, default=24)
     ap.add_argument("--uneven-cells", type=int, default=262144,
                     help="cells of the uneven-stiffness leg (0 skips it); it runs the timed step count")
+    ap.add_argument("--phase-timers", action="store_true",
+                    help="load libfastode_hip_timed.so (build.py --phase-timers) and report the batched "
+                         "kernel's cycles per phase of a cell-attempt over the timed intervals")
     ap.add_argument("--ts1-cells", type=int, default=65536,
                     help="cells of the TS1 leg (the reference's real mechanism; 0 skips it)")
     args = ap.parse_args()
@@ -277,6 +304,8 @@ def main():
               "across GPUs with no exchange, so the figure is N times the one-GPU rate", file=sys.stderr)
         sys.exit(2)
 
+    if args.phase_timers:
+        os.environ["FASTODE_AMD_LIB"] = "libfastode_hip_timed.so"
     import torch
     import __graft_entry__
     fa = __graft_entry__.load_package()
@@ -309,10 +338,11 @@ def main():
     progress(f"workload {wl.name}: {wl.C} cells, {K} timed intervals")
     tuned = None
     if not args.no_tune:
-        # autotuner: the first 16,384 cells per candidate (a start-up interval,
-        # then a timed steady one)
+        # autotuner: the first 8,192 cells per candidate (a start-up interval,
+        # then a timed steady one; the 16,384 cells that two-cells-per-CU
+        # candidates need when the register tile allows them)
         from fastode_amd.autotune import tune
-        n_s = min(wl.C, 16384)
+        n_s = min(wl.C, 16384 if wl.n <= 120 else 8192)
         t_tune = time.perf_counter()
         ch = tune(mech, k_dev[:n_s] if k_dev.shape[0] > 1 else k_dev, wl.Y0[:n_s], wl.dt, wl.atol, wl.rtol,
                   tableau=tab, kmul=wl.kmul(0), mass=wl.mass, use_cache=False)
@@ -325,7 +355,8 @@ def main():
     plan = mechanism_plan(mech).plan
     stage = {"auto": None, "yes": True, "no": False}[args.stage_tables]
     run, wall, launch_ms, counts, rec, cells_per_group = timed_run(
-        fa, torch, wl, tab, system, K, args.warmup, args.blocks_per_cu, not args.no_order, stage)
+        fa, torch, wl, tab, system, K, args.warmup, args.blocks_per_cu, not args.no_order, stage,
+        phase_timers=args.phase_timers)
     kernel_s = sum(launch_ms) / 1e3
     progress(f"timed: {wall:.2f} s wall, launches {[round(x, 1) for x in launch_ms]} ms")
     fails = int((rec["fails"] > 0).sum())
@@ -361,6 +392,8 @@ def main():
                 "fp64_tflops": round(alg_flops / kernel_s / 1e12, 3),
                 "fp64_frac": round(alg_flops / kernel_s / (FP64_PEAK_TFLOPS * 1e12), 4),
                 "us_per_cell_attempt": round(1e6 * kernel_s / cell_attempts, 4)}
+    if args.phase_timers:
+        roofline["phase_timers"] = run.phase_report
     prof = counter_profile(wl.name)
     if prof is not None:
         roofline["measured_binding"] = prof.get("binding")

@@ -6,6 +6,7 @@ raises at import, so that no product path can silently run without it.
 from __future__ import annotations
 
 import ctypes as ct
+import os
 from pathlib import Path
 
 import numpy as np
@@ -15,10 +16,19 @@ import numpy as np
 import torch  # noqa: F401
 
 HERE = Path(__file__).resolve().parent
-LIB_PATH = HERE / "libfastode_hip.so"
+# the product library, or (FASTODE_AMD_LIB=libfastode_hip_timed.so) the same
+# code built with the batched kernel's phase timers
+LIB_NAMES = ("libfastode_hip.so", "libfastode_hip_timed.so")
+_name = os.environ.get("FASTODE_AMD_LIB", LIB_NAMES[0])
+if _name not in LIB_NAMES:
+    raise ImportError(f"FASTODE_AMD_LIB must be one of {LIB_NAMES}")
+LIB_PATH = HERE / _name
+PHASES = ("load", "rates", "jacobian", "assemble", "sparse_factor", "schur", "core_factor", "inverses",
+          "stage_setup", "sparse_solve", "core_solve", "combine", "control")
 
 if not LIB_PATH.exists():
-    raise ImportError(f"{LIB_PATH} is not built: run __graft_entry__.build() (hipcc, gfx950)")
+    raise ImportError(f"{LIB_PATH} is not built: run __graft_entry__.build() (hipcc, gfx950)"
+                      + (" with phase timers: build.py --phase-timers" if _name != LIB_NAMES[0] else ""))
 
 lib = ct.CDLL(str(LIB_PATH))
 
@@ -129,6 +139,7 @@ _SIGS = {
     "fode_batched_interval": (i32, [vp, vp, P(CtrlParams), P(BatchDesc), i32, vp]),
     "fode_batched_lds_bytes": (i32, [vp, vp, i32, P(ct.c_int64)]),
     "fode_batched_interval_noalloc": (i32, [vp, vp, P(CtrlParams), P(BatchDesc), i32, vp]),
+    "fode_phase_timers": (i32, [P(ct.c_uint64), i32, i32]),
     "fode_batched_graph_create": (i32, [vp, vp, P(CtrlParams), P(BatchDesc), i32, P(vp)]),
     "fode_graph_launch": (i32, [vp, vp]),
     "fode_graph_destroy": (i32, [vp]),

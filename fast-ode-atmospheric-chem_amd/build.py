@@ -25,15 +25,18 @@ FLAGS = [
 ]
 
 
-def build(verbose=False) -> Path:
+def build(verbose=False, phase_timers=False) -> Path:
     """Each source to an object in parallel (the batched kernels dominate),
-    then one shared library."""
+    then one shared library.  phase_timers: the same code with the batched
+    kernel's phase timers (-DFODE_PHASE_TIMERS), as libfastode_hip_timed.so."""
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    lib = LIB.with_name("libfastode_hip_timed.so") if phase_timers else LIB
+    extra = ["-DFODE_PHASE_TIMERS"] if phase_timers else []
     with tempfile.TemporaryDirectory(prefix="fode_build_") as tmpdir:
         objs, procs = [], []
         for src in SRCS:
             obj = Path(tmpdir) / (src.stem + ".o")
-            cmd = [hipcc, *FLAGS, "-I", str(HERE.parent / "include"), "-c", str(src), "-o", str(obj)]
+            cmd = [hipcc, *FLAGS, *extra, "-I", str(HERE.parent / "include"), "-c", str(src), "-o", str(obj)]
             if verbose:
                 print(" ".join(cmd), file=sys.stderr)
             objs.append(obj)
@@ -41,14 +44,14 @@ def build(verbose=False) -> Path:
         failed = [cmd for cmd, pr in procs if pr.wait() != 0]
         if failed:
             raise subprocess.CalledProcessError(1, failed[0])
-        tmp = LIB.with_suffix(".so.tmp")
+        tmp = lib.with_suffix(".so.tmp")
         cmd = [hipcc, "-shared", "-fPIC", f"--offload-arch={ARCH}", *map(str, objs), "-o", str(tmp)]
         if verbose:
             print(" ".join(cmd), file=sys.stderr)
         subprocess.run(cmd, check=True)
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib)
+    return lib
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
+    print(build(verbose=True, phase_timers="--phase-timers" in sys.argv[1:]))

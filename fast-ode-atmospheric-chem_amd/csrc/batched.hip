@@ -24,6 +24,11 @@ using namespace fode;
 #define FODE_NOALLOC_SCRATCH_BYTES (32u << 20)
 static __device__ __align__(256) double scratch[FODE_NOALLOC_SCRATCH_BYTES / sizeof(double)];
 
+#ifdef FODE_PHASE_TIMERS
+// Cycle sums per phase over every workgroup of every timed launch.
+static __device__ __align__(256) unsigned long long phase_cycles[PH_COUNT];
+#endif
+
 namespace {
 
 __global__ void k_reset_queue(int* q) {
@@ -79,6 +84,12 @@ int prepare(void* plan, void* mech, const fode_ctrl_params* p, const fode_batch_
   A.work_by_block = 0;
   A.work_half = 0;
   A.max_blocks = 0;
+  A.timers = nullptr;
+#ifdef FODE_PHASE_TIMERS
+  void* t = nullptr;
+  if (hipGetSymbolAddress(&t, HIP_SYMBOL(phase_cycles)) != hipSuccess) return bfail(w + ": phase timers");
+  A.timers = static_cast<unsigned long long*>(t);
+#endif
   return 0;
 }
 
@@ -167,6 +178,25 @@ int fode_batched_interval(void* plan, void* mech, const fode_ctrl_params* p, con
   if (prepare(plan, mech, p, b, "fode_batched_interval", A)) return -1;
   if (!b->work || !b->queue) return bfail("fode_batched_interval: null array");
   return launch(A, blocks_per_cu, reinterpret_cast<hipStream_t>(stream));
+}
+
+int fode_phase_timers(uint64_t* out, int n, int reset) {
+#ifdef FODE_PHASE_TIMERS
+  if (!out || n < 0) return bfail("fode_phase_timers: bad argument");
+  unsigned long long host[PH_COUNT];
+  hipError_t e = hipMemcpyFromSymbol(host, HIP_SYMBOL(phase_cycles), sizeof(host));
+  if (e != hipSuccess) return bfail(std::string("fode_phase_timers: ") + hipGetErrorString(e));
+  for (int i = 0; i < n && i < PH_COUNT; ++i) out[i] = host[i];
+  if (reset) {
+    const unsigned long long zero[PH_COUNT] = {};
+    e = hipMemcpyToSymbol(HIP_SYMBOL(phase_cycles), zero, sizeof(zero));
+    if (e != hipSuccess) return bfail(std::string("fode_phase_timers: ") + hipGetErrorString(e));
+  }
+  return PH_COUNT;
+#else
+  (void)out; (void)n; (void)reset;
+  return bfail("fode_phase_timers: this library was built without FODE_PHASE_TIMERS");
+#endif
 }
 
 int fode_batched_interval_noalloc(void* plan, void* mech, const fode_ctrl_params* p, const fode_batch_desc* b,

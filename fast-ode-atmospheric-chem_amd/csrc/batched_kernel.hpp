@@ -34,6 +34,7 @@ struct BatchArgs {
   int work_by_block;
   long work_half;
   int max_blocks;   // > 0: launch at most this many workgroups (work slots)
+  unsigned long long* timers;   // phase cycle sums (timed builds), else null
 };
 
 // A group's scratch rows (half 0: new states / Newton iterates; half 1: the
@@ -71,9 +72,12 @@ __device__ __forceinline__ double cell_attempt(const BatchArgs& A, const Tables&
   if (A.b.k_stride != 0)
     for (int j = tid; j < M.R; j += NT) V.kc[j] = kg[j] * A.b.kmul[j];
   __syncthreads();
+  FODE_PHASE(PH_LOAD);
   cell_stages<TILE>(A.P, T, M, A.p, A.b.mass, L, V, g.H, g.alpha, ynew_out);
-  if (step_change_rows(A.p)) return cell_err_sum_dae(V.y, V.w, V.fr, A.b.atol, A.b.rtol, n, V.red, A.b.mass);
-  return cell_err_sum(V.y, V.w, V.fr, A.b.atol, A.b.rtol, n, V.red);
+  const double e = step_change_rows(A.p) ? cell_err_sum_dae(V.y, V.w, V.fr, A.b.atol, A.b.rtol, n, V.red, A.b.mass)
+                                         : cell_err_sum(V.y, V.w, V.fr, A.b.atol, A.b.rtol, n, V.red);
+  FODE_PHASE(PH_COMBINE);
+  return e;
 }
 
 // Maximum over the workgroup of one double per thread, to every thread.
@@ -288,6 +292,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
   __shared__ fode_group_ctrl gs;
   const int tid = threadIdx.x;
   const fode_batch_desc& B = A.b;
+  phase_start();
   if (B.k_stride == 0)
     for (int j = tid; j < A.M.R; j += NT) V.kc[j] = B.k[j] * B.kmul[j];
   for (;;) {
@@ -346,6 +351,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
         for (long i = tid; i < N; i += NT) Y[i] = W[i];
       }
       __syncthreads();
+      FODE_PHASE(PH_CONTROL);
     }
     // clamp (fastode/solvers.py:89-97) and record the call's status
     if (gs.status != FODE_PROJECTION_FAILED) {
@@ -365,7 +371,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
       if (B.nacc) B.nacc[group] = nrec;
     }
     __syncthreads();
+    FODE_PHASE(PH_CONTROL);
   }
+  phase_flush(A.timers);
 }
 
 // np.maximum(a, b): NaN from either side propagates.

@@ -201,9 +201,12 @@ __device__ __forceinline__ void cell_stages(const PlanDev& P, const Tables& T, c
   // attempt: they are the same values a retry reuses.
   rates_into(M, V.y, V.kc, V.rd);
   forcing_into(M, V.y, V.rd, V.f0);
+  FODE_PHASE(PH_RATES);
   jacobian_into(M, V);
+  FODE_PHASE(PH_JACOBIAN);
   double a[TILE][TILE];
   assemble<TILE>(P, JLds{V.jv}, alpha, mass, false, L.lds, a);
+  FODE_PHASE(PH_ASSEMBLE);
   factor_cell<TILE>(P, T, L, a);
   // Stages (fastode/solvers.py:54-86).
   double* vec = L.lds + P.vec;
@@ -244,6 +247,7 @@ __device__ __forceinline__ void cell_stages(const PlanDev& P, const Tables& T, c
     solve_cell<TILE>(P, T, L, a);
     for (int q = tid; q < n; q += NT) V.K[r * n + P.perm[q]] = vec[q];
     __syncthreads();
+    FODE_PHASE(PH_STAGE_SETUP);
   }
   // Ynew and the embedded error; E kept in fr.
   for (int i = tid; i < n; i += NT) {

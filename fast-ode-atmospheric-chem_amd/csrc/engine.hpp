@@ -15,6 +15,7 @@
 #include "chem.hpp"
 #include "common.hpp"
 #include "control.hpp"
+#include "timers.hpp"
 
 namespace fode {
 
@@ -572,9 +573,13 @@ __device__ __forceinline__ void factor_cell(const PlanDev& P, const Tables& T, c
                                    double (&a)[TILE][TILE]) {
   __syncthreads();
   run_tasks<false>(T.f_tasks, T.f_ptr, P.f_nst, T.f_pairs, L.lds);
+  FODE_PHASE(PH_SPARSE_FACTOR);
   schur<TILE>(P, T, L.lds, L.pb, a);
+  FODE_PHASE(PH_SCHUR);
   core_factor<TILE>(a, L);
+  FODE_PHASE(PH_CORE_FACTOR);
   diag_inverses<TILE>(a, L);
+  FODE_PHASE(PH_INVERSES);
 }
 
 // Solve with the cell's factor; the right-hand side is already in the LDS
@@ -584,9 +589,13 @@ __device__ __forceinline__ void solve_cell(const PlanDev& P, const Tables& T, co
                                   const double (&a)[TILE][TILE]) {
   double* vec = L.lds + P.vec;
   __syncthreads();
+  FODE_PHASE(PH_STAGE_SETUP);
   run_tasks<true>(T.sf_tasks, T.sf_ptr, P.sf_nst, T.s_pairs, L.lds);
+  FODE_PHASE(PH_SPARSE_SOLVE);
   core_solve<TILE>(a, L, vec + P.c);
+  FODE_PHASE(PH_CORE_SOLVE);
   run_tasks<true>(T.sb_tasks, T.sb_ptr, P.sb_nst, T.s_pairs, L.lds);
+  FODE_PHASE(PH_SPARSE_SOLVE);
 }
 
 }  // namespace fode

@@ -260,6 +260,16 @@ int fode_batched_interval(void* plan, void* mech, const fode_ctrl_params* p, con
 int fode_batched_interval_noalloc(void* plan, void* mech, const fode_ctrl_params* p, const fode_batch_desc* b,
                                   int blocks_per_cu, void* stream);
 
+/* Phase timers of the batched kernel: the cycles its workgroups spent per
+ * phase of a cell-attempt (load, rates, Jacobian, assembly, sparse factor,
+ * Schur update, core factor, block inverses, stage set-up, sparse solves,
+ * core solves, combination, control), summed over every launch since the
+ * last reset, into out[0..n); reset clears them.  Returns the number of
+ * phases, or -1 in a library built without FODE_PHASE_TIMERS (the product
+ * build: `build.py --phase-timers` makes libfastode_hip_timed.so).  No
+ * counterpart in the reference: a measurement facility.  Synchronous. */
+int fode_phase_timers(uint64_t* out, int n, int reset);
+
 /* LDS bytes one workgroup of the batched kernel uses for this plan/mechanism. */
 int fode_batched_lds_bytes(void* plan, void* mech, int stages, int64_t* bytes);
 
