@@ -3,8 +3,8 @@
     (M/(h gamma) - J) K_r = F(Y + sum_s a_rs K_s) + (M/h) sum_s c_rs K_s,
     Y_new = Y + sum_r m_r K_r,    error = sum_r e_r K_r.
 
-The coefficients are Sandu et al. (1997) as KPP prints them; the device reads
-them from a small constant block built by ``Tableau.device_block``.
+The coefficients are Sandu et al. (1997) as KPP prints them; the solvers copy
+them into the device parameter block (``solvers.ctrl_params``).
 """
 from __future__ import annotations
 
@@ -43,34 +43,6 @@ class Tableau:
             prev = np.append(self.a[r - 1, : r - 1], 0.0)
             flags.append(not (np.array_equal(self.a[r, :r], prev) and self.a[r, r - 1] == 0.0))
         return flags
-
-    def standard_form(self):
-        """(A, Gamma, b, bhat) of the standard form (tableaus.py:44-51)."""
-        S = self.stages
-        Gam = np.linalg.inv(np.eye(S) / self.gamma - self.c)
-        b = Gam.T @ self.m
-        return self.a @ Gam, Gam, b, b - Gam.T @ self.e
-
-    def index1(self, q: int) -> float:
-        A, Gam, b, _ = self.standard_form()
-        return float(b @ np.linalg.solve(A + Gam, A.sum(axis=1) ** q))
-
-    def device_block(self) -> np.ndarray:
-        """Packed float64 block read by the kernels:
-        [S, gamma, k, a (4x4), c (4x4), m (4), e (4), new_forcing (4)]."""
-        S = self.stages
-        out = np.zeros(3 + 16 + 16 + 4 + 4 + 4)
-        out[0], out[1], out[2] = S, self.gamma, self.k
-        a = np.zeros((MAX_STAGES, MAX_STAGES))
-        c = np.zeros((MAX_STAGES, MAX_STAGES))
-        a[:S, :S] = self.a
-        c[:S, :S] = self.c
-        out[3:19] = a.ravel()
-        out[19:35] = c.ravel()
-        out[35:35 + S] = self.m
-        out[39:39 + S] = self.e
-        out[43:43 + S] = np.asarray(self.new_forcing, dtype=float)
-        return out
 
 
 def _lower(S, flat):

@@ -304,13 +304,16 @@ def run_scenario(scenario: Scenario, solver, cells: Cells, ts1: TS1, rtol, atol,
 
 
 def run_scenario_batched(scenario: Scenario, solver, cells: Cells, ts1: TS1, rtol, atol, group_size=32,
-                         constrained=(), warmup_steps=0, steps=None, graph=True, on_step=None):
+                         constrained=(), warmup_steps=0, steps=None, graph=True, on_step=None, substeps=1):
     """The scenario with every group in one persistent launch per interval.
     solver: ``BatchedAlgorithm`` / ``BatchedRestartingLoop`` /
     ``BatchedBackwardEuler`` (its group_size is used).  Rate constants are
     computed once for all cells (the same per-cell arithmetic as the group
     loop's), moved to the device, and the photolysis columns rewritten on the
-    device from the second interval on.  Returns what ``run_scenario``
+    device from the second interval on.  substeps > 1 calls the solver that
+    many times per interval on equal pieces of it, the photolysis held over
+    the interval (the sub-stepped backward Euler of
+    experiments/be_substeps.py:33-52).  Returns what ``run_scenario``
     returns, plus the runner (device state, per-group Memory and Counts)."""
     import torch
 
@@ -332,7 +335,7 @@ def run_scenario_batched(scenario: Scenario, solver, cells: Cells, ts1: TS1, rto
     scale_dev = as_device(scale)
     f_dev = torch.zeros(1, dtype=torch.float64, device=k_dev.device)
     system = ChemSystem(ts1.mech, k_dev, mass=mass, dense=False)
-    runner = solver.runner(system, cells.Y, scenario.dt, atol_v, rtol, graph=graph)
+    runner = solver.runner(system, cells.Y, scenario.dt / substeps, atol_v, rtol, graph=graph)
     runner.set_kmul(None)
     snapshots = {}
     statuses = []
@@ -344,10 +347,14 @@ def run_scenario_batched(scenario: Scenario, solver, cells: Cells, ts1: TS1, rto
                                              L.ptr(base_dev), L.ptr(scale_dev), L.ptr(f_dev),
                                              L.stream_handle()), "fode_scale_columns")
         first = False
-        runner.step_graph_only()
-        st = runner.memory.read()["status"]
-        for g in np.flatnonzero(st != L.CONVERGED):
-            statuses.append((int(g) * group_size, step, L.STATUS[int(st[g])]))
+        failed = {}
+        for _ in range(substeps):
+            runner.step_graph_only()
+            st = runner.memory.read()["status"]
+            for g in np.flatnonzero(st != L.CONVERGED):
+                failed[int(g)] = L.STATUS[int(st[g])]
+        for g, status in sorted(failed.items()):
+            statuses.append((g * group_size, step, status))
         if on_step is not None:
             Yh = runner.Y.cpu().numpy()
             for g0 in range(0, cells.C, group_size):

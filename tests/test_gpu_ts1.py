@@ -203,3 +203,35 @@ def test_ts1_constrained_families_match_recorded_runs(rtol, form, solver, batche
     if fam:
         assert abs(min(per[x] for x in fam if x in per) - float(ref["constrained_min_digits"])) <= 0.01
     assert len(statuses) == int(ref["failed_calls"])
+
+
+def be_substeps_row(scenario, n):
+    with open(GOLD / "be_substeps.csv", newline="") as f:
+        for r in csv.DictReader(f):
+            if r["scenario"] == scenario and int(r["substeps"]) == n:
+                return r
+    raise KeyError((scenario, n))
+
+
+@pytest.mark.parametrize("scenario,n", [("tier1-30s", 4), ("tier1-30s", 16), ("tier1-30s", 64),
+                                        ("diurnal-10min", 4), ("diurnal-10min", 16)])
+def test_batched_substepped_backward_euler_matches_recorded_runs(scenario, n):
+    """Backward Euler called n times per interval on equal pieces
+    (experiments/be_substeps.py): Newton passes per group-interval exact,
+    digits to 0.01 of reference-results/ts1/be_substeps.csv."""
+    import fastode_amd as fa
+    import fastode_amd.ts1 as T
+    m = ts1()
+    scn = T.Scenario.from_npz(GOLD / f"{scenario}.npz")
+    cl = scn.make_cells(m, which=range(64))
+    snaps, counts, statuses, _ = T.run_scenario_batched(scn, fa.BatchedBackwardEuler(), cl, m, 1e-6, 1e-3,
+                                                        substeps=n)
+    D, per, gate = T.score(snaps, range(64), m.species, scn.reference())
+    ref = be_substeps_row(scenario, n)
+    per_int = 1.0 / scn.steps / 2
+    print(f"be x{n} {scenario}: newton/interval {counts.newton * per_int}, digits {D:.4f} (recorded {ref['digits']})")
+    assert round(counts.newton * per_int, 2) == float(ref["newton_per_interval"])
+    assert (counts.failures, counts.unconverged, len(statuses)) == (int(ref["failures"]), int(ref["unconverged"]),
+                                                                    int(ref["failed_calls"]))
+    assert abs(D - float(ref["digits"])) <= 0.01
+    assert min(per, key=per.get) == ref["worst_species"]
