@@ -169,6 +169,9 @@ class BatchedRunner:
                                                     self._blocks_per_cu(balg, system), ct.byref(h)),
                     "fode_batched_graph_create")
             self.graph = h
+            # held here: module globals may be gone when a runner is
+            # collected at interpreter exit
+            self._graph_destroy = L.lib.fode_graph_destroy
 
     @staticmethod
     def _blocks_per_cu(balg, system):
@@ -238,5 +241,5 @@ class BatchedRunner:
     def __del__(self):
         g = getattr(self, "graph", None)
         if g:
-            L.lib.fode_graph_destroy(g)
+            self._graph_destroy(g)
             self.graph = None

@@ -47,11 +47,12 @@ class DeviceMechanism:
         L.check(L.lib.fode_mech_create(ct.byref(d), ct.byref(h)), "fode_mech_create")
         self.handle = h
         self._keep = arrs
+        self._destroy = L.lib.fode_mech_destroy   # module globals may be gone at interpreter exit
 
     def __del__(self):
         h = getattr(self, "handle", None)
         if h:
-            L.lib.fode_mech_destroy(h)
+            self._destroy(h)
             self.handle = None
 
 
@@ -84,11 +85,12 @@ class DevicePlan:
         self.handle = h
         self.lu_len = lu_len.value
         self._keep = arrs
+        self._destroy = L.lib.fode_plan_destroy
 
     def __del__(self):
         h = getattr(self, "handle", None)
         if h:
-            L.lib.fode_plan_destroy(h)
+            self._destroy(h)
             self.handle = None
 
 

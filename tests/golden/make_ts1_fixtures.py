@@ -8,7 +8,7 @@ text; nothing of the reference is imported or run).
   reference_final.csv and, where present, reference_snapshots.csv, each CSV
   as (header, float64 rows) parsed the way fastode/ts1.py:106-111 parses
   them (float() of every field).
-* tests/golden/ts1/{runs,be_substeps,rejections}.csv and ts1_families.csv:
+* tests/golden/ts1/{runs,atol,be_substeps,rejections}.csv and ts1_families.csv:
   the reference's recorded results (reference-results/ts1/*.csv,
   reference-results/dae/ts1_families.csv), copied verbatim.
 
@@ -63,7 +63,8 @@ def main():
             arrays["snap_header"], arrays["snaps"] = table(d / "reference_snapshots.csv")
         np.savez_compressed(OUT / f"{d.name}.npz", **arrays)
         print(d.name, {k: getattr(v, "shape", v) for k, v in arrays.items()})
-    for src in ("reference-results/ts1/runs.csv", "reference-results/ts1/be_substeps.csv",
+    for src in ("reference-results/ts1/runs.csv", "reference-results/ts1/atol.csv",
+                "reference-results/ts1/be_substeps.csv",
                 "reference-results/ts1/rejections.csv", "reference-results/dae/ts1_families.csv"):
         shutil.copyfile(REF / src, OUT / Path(src).name)
 
