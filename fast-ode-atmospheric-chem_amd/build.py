@@ -53,5 +53,28 @@ def build(verbose=False, phase_timers=False) -> Path:
     return lib
 
 
+PROBES = [HERE / "csrc" / "probes" / "core_factor_probe.hip"]
+
+
+def build_probes(verbose=False) -> list:
+    """The measurement probes (standalone executables next to their sources,
+    not part of the library): core_factor_probe times the core factor, four
+    restructurings of it checked bit for bit against it, its throughput at
+    one to four workgroups per CU, and the latency floors of a pivot round."""
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    out = []
+    for src in PROBES:
+        exe = src.with_suffix("")
+        cmd = [hipcc, *FLAGS, "-I", str(HERE.parent / "include"), str(src), "-o", str(exe)]
+        if verbose:
+            print(" ".join(cmd), file=sys.stderr)
+        subprocess.run(cmd, check=True)
+        out.append(exe)
+    return out
+
+
 if __name__ == "__main__":
-    print(build(verbose=True, phase_timers="--phase-timers" in sys.argv[1:]))
+    if "--probes" in sys.argv[1:]:
+        print(*build_probes(verbose=True), sep="\n")
+    else:
+        print(build(verbose=True, phase_timers="--phase-timers" in sys.argv[1:]))
