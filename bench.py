@@ -13,7 +13,9 @@ HBM before timing starts.
 The JSON line also carries the per-group plain form (one launch sequence
 per group and protocol call, the reference's call structure) measured on a
 sample of groups over the same intervals, the HBM roofline of the interval
-kernel and the CPU oracle's rate on the box's cores.  The adjoint leg times
+kernel, the legs of the other workloads (262,144 cells of uneven stiffness,
+TS1 on 65,536 cells, the constrained RODAS3 workload on 65,536 cells) over
+the same number of intervals, and the CPU oracle's rate on the box's cores.  The adjoint leg times
 the gradient of sum w_s log(Y_final + 1e-17) with respect to log k and Y0
 on 16,384 cells over 24 intervals against the plain batched forward of the
 same intervals ("adjoint_over_forward").
@@ -298,6 +300,8 @@ def main():
                          "silent for 180 s on the GPU twice, with no cause found in the code")
     ap.add_argument("--ts1-cells", type=int, default=65536,
                     help="cells of the TS1 leg (the reference's real mechanism; 0 skips it)")
+    ap.add_argument("--constrained-cells", type=int, default=65536,
+                    help="cells of the constrained RODAS3 leg (100 species + 3 conservation rows; 0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: the groups shard "
@@ -455,6 +459,9 @@ def main():
     if args.ts1_cells > 0 and args.workload == "diurnal":
         progress("ts1 leg")
         out["ts1"] = ts1_leg(fa, torch, args.ts1_cells, K, args)
+    if args.constrained_cells > 0 and args.workload == "diurnal":
+        progress("constrained leg")
+        out["constrained"] = constrained_leg(fa, torch, W, args.constrained_cells, K, args)
     if args.adjoint_cells > 0:
         progress("adjoint leg")
         out["adjoint"] = adjoint_leg(fa, torch, W, args.adjoint_cells, args.adjoint_intervals,
@@ -540,6 +547,41 @@ def ts1_leg(fa, torch, C, K, args, scenario="diurnal-10min"):
            "failed_groups": int((rec["fails"] > 0).sum()), "autotuned": tuned,
            "lu": {"nnz_J": m.mech.nnz, "core": m.mech._plan.plan.m, "tile": m.mech._plan.plan.tile},
            "note": "the scenario's 64 recorded cells tiled; atol 1e-17, rtol 1e-3"}
+    del run
+    return out
+
+
+def constrained_leg(fa, torch, W, C, K, args):
+    """The constrained workload (100 species, 250 reactions, 3 conservation
+    rows replacing species rows, RODAS3 with the projection at every call)
+    on C cells over the timed step count, autotuned on its first 16,384
+    cells, one graph replay per interval from the initial state."""
+    from fastode_amd.system import ChemSystem
+    dev = torch.device("cuda", 0)
+    wl = W.constrained100(C=C, n_intervals=K)
+    k_dev = k_resident(wl, torch, dev)
+    tuned = None
+    if not args.no_tune:
+        from fastode_amd.autotune import tune
+        n_s = min(C, 16384)
+        ch = tune(wl.mech, k_dev[:n_s] if k_dev.shape[0] > 1 else k_dev, wl.Y0[:n_s], wl.dt, wl.atol, wl.rtol,
+                  tableau=fa.RODAS3, kmul=wl.kmul(0), mass=wl.mass, use_cache=False)
+        tuned = {"c": ch.c, "tile": ch.tile, "stage_tables": ch.stage_tables,
+                 "blocks_per_cu": ch.blocks_per_cu or "max", "us_per_cell_attempt": round(ch.us_per_cell_attempt, 3)}
+        progress(f"constrained tuned: {tuned}")
+    system = ChemSystem(wl.mech, k_dev, mass=wl.mass)
+    run, wall, launch_ms, counts, rec, cpg = timed_run(fa, torch, wl, fa.RODAS3, system, K, 0, args.blocks_per_cu,
+                                                       not args.no_order)
+    attempts = float((counts[:, 0] * cpg).sum())
+    out = {"cells": C, "species": wl.n, "reactions": wl.R, "constraints": int(len(wl.mech.constraint_rows)),
+           "intervals": K, "dt_s": wl.dt, "tableau": wl.tableau, "value": round(C * K / wall, 1),
+           "unit": "cell-intervals/s", "ms_per_interval": round(1e3 * wall / K, 3),
+           "avg_launch_ms": round(sum(launch_ms) / K, 3),
+           "attempts_per_group_interval": round(float(counts[:, 0].sum()) / (run.ngroups * K), 3),
+           "projection_passes_per_group_interval": round(float(counts[:, 10].sum()) / (run.ngroups * K), 3),
+           "us_per_cell_attempt": round(1e3 * sum(launch_ms) / attempts, 4),
+           "failed_groups": int((rec["fails"] > 0).sum()), "autotuned": tuned,
+           "lu": {"nnz_J": wl.mech.nnz, "core": wl.mech._plan.plan.m, "tile": wl.mech._plan.plan.tile}}
     del run
     return out
 
