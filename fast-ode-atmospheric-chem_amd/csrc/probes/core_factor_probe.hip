@@ -22,6 +22,8 @@
 //         row broadcasts, permlane swaps), the column panel by in-row
 //         substitution, the row panel through one LDS transpose, one
 //         rank-16 trailing pass; three barriers per block;
+//       5 quad: four pivots per barrier instead of two, every thread
+//         forming the 4x4 pivot block's multipliers itself;
 //   * the library's factor with two, three and four workgroups per CU
 //     (throughput per factorization);
 //   * latency floors of one pivot-pair round: a barrier, an LDS
@@ -377,6 +379,104 @@ __device__ inline void blocked_block(double (&a)[TILE][TILE], double* D, double*
   __syncthreads();
 }
 
+
+// ------------------------------------------------------------------ 5. quad
+// Four pivots per barrier: the columns and rows of pivots k0..k0+3 (updated
+// through k0-1) are published together, and every thread forms the 4x4
+// pivot block's multipliers and reciprocals itself, in the order the pair
+// scheme forms them.
+template <int TILE, int NB>
+__device__ inline void publish_quad(const double (&a)[TILE][TILE], int kk, int nb, double* lbuf, double* ubuf,
+                                    double* rcs, int tr, int tc) {
+  constexpr int MP = G * TILE;
+  double* c = lbuf + 4 * nb * MP;
+  double* r = ubuf + 4 * nb * MP;
+  const int dc = tc - kk, dr = tr - kk;
+  if (dc >= 0 && dc < 4) {
+#pragma unroll
+    for (int ti = NB; ti < TILE; ++ti) c[dc * MP + ti * G + tr] = a[ti][NB];
+  }
+  if (dr >= 0 && dr < 4) {
+#pragma unroll
+    for (int tj = NB; tj < TILE; ++tj) r[dr * MP + tj * G + tc] = a[NB][tj];
+    if (dr == 0 && dc == 0) rcs[NB * G + kk] = pivot_rcp(a[NB][NB]);
+  }
+}
+
+template <int TILE, int B>
+__device__ inline void quad_block(double (&a)[TILE][TILE], double* lbuf, double* ubuf, double* rcs, int tr, int tc) {
+  constexpr int MP = G * TILE;
+  double rcown = 0.0;
+#pragma nounroll
+  for (int kk = 0; kk < G; kk += 4) {
+    const int k0 = B * G + kk, buf = (k0 >> 2) & 1;
+    const double* c0 = lbuf + 4 * buf * MP;
+    const double* c1 = c0 + MP;
+    const double* c2 = c1 + MP;
+    const double* c3 = c2 + MP;
+    const double* r0 = ubuf + 4 * buf * MP;
+    const double* r1 = r0 + MP;
+    const double* r2 = r1 + MP;
+    const double* r3 = r2 + MP;
+    const int q = B * G + kk;
+    // the 4x4 pivot block
+    const double rc0 = rcs[k0];
+    const double l10 = c0[q + 1] * rc0, l20 = c0[q + 2] * rc0, l30 = c0[q + 3] * rc0;
+    const double u01 = r0[q + 1], u02 = r0[q + 2], u03 = r0[q + 3];
+    const double rc1 = pivot_rcp(fma(-l10, u01, r1[q + 1]));
+    const double u12 = fma(-l10, u02, r1[q + 2]), u13 = fma(-l10, u03, r1[q + 3]);
+    const double l21 = fma(-l20, u01, c1[q + 2]) * rc1, l31 = fma(-l30, u01, c1[q + 3]) * rc1;
+    const double rc2 = pivot_rcp(fma(-l21, u12, fma(-l20, u02, r2[q + 2])));
+    const double u23 = fma(-l21, u13, fma(-l20, u03, r2[q + 3]));
+    const double l32 = fma(-l31, u12, fma(-l30, u02, c2[q + 3])) * rc2;
+    const double rc3 = pivot_rcp(fma(-l32, u23, fma(-l31, u13, fma(-l30, u03, r3[q + 3]))));
+    rcown = (tc == kk) ? rc0 : ((tc == kk + 1) ? rc1 : ((tc == kk + 2) ? rc2 : ((tc == kk + 3) ? rc3 : rcown)));
+    double v0[TILE], v1[TILE], v2[TILE], v3[TILE];
+#pragma unroll
+    for (int tj = B; tj < TILE; ++tj) {
+      const double w0 = r0[tj * G + tc];
+      const double w1 = fma(-l10, w0, r1[tj * G + tc]);
+      const double w2 = fma(-l21, w1, fma(-l20, w0, r2[tj * G + tc]));
+      const double w3 = fma(-l32, w2, fma(-l31, w1, fma(-l30, w0, r3[tj * G + tc])));
+      v0[tj] = w0;
+      v1[tj] = w1;
+      v2[tj] = w2;
+      v3[tj] = w3;
+    }
+    v0[B] = (tc > kk) ? v0[B] : 0.0;
+    v1[B] = (tc > kk + 1) ? v1[B] : 0.0;
+    v2[B] = (tc > kk + 2) ? v2[B] : 0.0;
+    v3[B] = (tc > kk + 3) ? v3[B] : 0.0;
+#pragma unroll
+    for (int ti = B; ti < TILE; ++ti) {
+      double x0 = c0[ti * G + tr] * rc0;
+      if (ti == B) x0 = (tr > kk) ? x0 : 0.0;
+      double x1 = fma(-x0, u01, c1[ti * G + tr]) * rc1;
+      if (ti == B) x1 = (tr > kk + 1) ? x1 : 0.0;
+      double x2 = fma(-x1, u12, fma(-x0, u02, c2[ti * G + tr])) * rc2;
+      if (ti == B) x2 = (tr > kk + 2) ? x2 : 0.0;
+      double x3 = fma(-x2, u23, fma(-x1, u13, fma(-x0, u03, c3[ti * G + tr]))) * rc3;
+      if (ti == B) x3 = (tr > kk + 3) ? x3 : 0.0;
+#pragma unroll
+      for (int tj = B; tj < TILE; ++tj) {
+        double t = fma(-x0, v0[tj], a[ti][tj]);
+        t = fma(-x1, v1[tj], t);
+        t = fma(-x2, v2[tj], t);
+        a[ti][tj] = fma(-x3, v3[tj], t);
+      }
+    }
+    if (kk + 4 < G) {
+      publish_quad<TILE, B>(a, kk + 4, buf ^ 1, lbuf, ubuf, rcs, tr, tc);
+    } else if constexpr (B + 1 < TILE) {
+      publish_quad<TILE, B + 1>(a, 0, buf ^ 1, lbuf, ubuf, rcs, tr, tc);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int ti = B; ti < TILE; ++ti)
+    if (ti != B || tr > tc) a[ti][B] = a[ti][B] * rcown;
+}
+
 template <int TILE, int V, int B>
 __device__ inline void variant_blocks(double (&a)[TILE][TILE], double* smem, PairRaw<TILE>& w, int tr, int tc) {
   if constexpr (B < TILE) {
@@ -388,12 +488,13 @@ __device__ inline void variant_blocks(double (&a)[TILE][TILE], double* smem, Pai
                                  tr, tc);
     if constexpr (V == 3) ahead_block<TILE, B, false>(a, w, smem, smem + 4 * MP, smem + 8 * MP, nullptr, tr, tc);
     if constexpr (V == 4) blocked_block<TILE, B>(a, smem, smem + NT, smem + 2 * NT, smem + 2 * NT + MP * PS, tr, tc);
+    if constexpr (V == 5) quad_block<TILE, B>(a, smem, smem + 8 * MP, smem + 16 * MP, tr, tc);
     variant_blocks<TILE, V, B + 1>(a, smem, w, tr, tc);
   }
 }
 
-// LDS (doubles): V1 9*MP + 2*MP*PS; V2 17*MP + 1; V3 9*MP; V4 2*NT + 2*MP*PS:
-// at most 6,624 at TILE 10, far below the request.
+// LDS (doubles): V1 9*MP + 2*MP*PS; V2 17*MP + 1; V3 9*MP; V4 2*NT + 2*MP*PS;
+// V5 17*MP: at most 6,624 at TILE 10, far below the request.
 template <int TILE, int V>
 __device__ inline void factor_variant(double (&a)[TILE][TILE], double* smem) {
   const int tid = threadIdx.x, tr = tid / G, tc = tid % G;
@@ -401,6 +502,9 @@ __device__ inline void factor_variant(double (&a)[TILE][TILE], double* smem) {
   PairRaw<TILE> w;
   if constexpr (V == 4) {
     smem[tid] = a[0][0];
+    __syncthreads();
+  } else if constexpr (V == 5) {
+    publish_quad<TILE, 0>(a, 0, 0, smem, smem + 8 * MP, smem + 16 * MP, tr, tc);
     __syncthreads();
   } else {
     double* rcs = smem + (V == 2 ? 16 : 8) * MP;
@@ -594,6 +698,7 @@ void tile_table(int cus, int reps) {
   variant_line<TILE, 2>(cus, reps, ref, "lookahead");
   variant_line<TILE, 3>(cus, reps, ref, "publish-first");
   variant_line<TILE, 4>(cus, reps, ref, "blocked");
+  variant_line<TILE, 5>(cus, reps, ref, "quad");
 }
 
 template <int TILE>
