@@ -8,7 +8,7 @@
 // diagonally dominant core of 16*TILE rows `reps` times; the shader clock
 // (s_memtime) brackets each factorization.  Reported per TILE (9: the
 // 200-species core, 5: the constrained 100-species core, 2: TS1):
-//   * the library's factor (variant 0) and four restructurings of it that
+//   * the library's factor (variant 0) and five restructurings of it that
 //     keep the fma sequence of every entry (each is checked bit for bit
 //     against variant 0 on the same matrices):
 //       1 deferred: the pivot block's cross updated pair by pair, the
