@@ -58,7 +58,7 @@ PROBES = [HERE / "csrc" / "probes" / "core_factor_probe.hip"]
 
 def build_probes(verbose=False) -> list:
     """The measurement probes (standalone executables next to their sources,
-    not part of the library): core_factor_probe times the core factor, five
+    not part of the library): core_factor_probe times the core factor, six
     restructurings of it checked bit for bit against it, its throughput at
     one to four workgroups per CU, and the latency floors of a pivot round."""
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -8,7 +8,7 @@
 // diagonally dominant core of 16*TILE rows `reps` times; the shader clock
 // (s_memtime) brackets each factorization.  Reported per TILE (9: the
 // 200-species core, 5: the constrained 100-species core, 2: TS1):
-//   * the library's factor (variant 0) and five restructurings of it that
+//   * the library's factor (variant 0) and six restructurings of it that
 //     keep the fma sequence of every entry (each is checked bit for bit
 //     against variant 0 on the same matrices):
 //       1 deferred: the pivot block's cross updated pair by pair, the
@@ -24,6 +24,8 @@
 //         rank-16 trailing pass; three barriers per block;
 //       5 quad: four pivots per barrier instead of two, every thread
 //         forming the 4x4 pivot block's multipliers itself;
+//       6 dual: two cells' cores per workgroup, factored with the library's
+//         pair scheme and sharing each pair's barrier (time per core);
 //   * the library's factor with two, three and four workgroups per CU
 //     (throughput per factorization);
 //   * latency floors of one pivot-pair round: a barrier, an LDS
@@ -494,7 +496,7 @@ __device__ inline void variant_blocks(double (&a)[TILE][TILE], double* smem, Pai
 }
 
 // LDS (doubles): V1 9*MP + 2*MP*PS; V2 17*MP + 1; V3 9*MP; V4 2*NT + 2*MP*PS;
-// V5 17*MP: at most 6,624 at TILE 10, far below the request.
+// V5 17*MP; V6 (two cores) 18*MP: at most 6,624 at TILE 10, far below the request.
 template <int TILE, int V>
 __device__ inline void factor_variant(double (&a)[TILE][TILE], double* smem) {
   const int tid = threadIdx.x, tr = tid / G, tc = tid % G;
@@ -517,6 +519,79 @@ __device__ inline void factor_variant(double (&a)[TILE][TILE], double* smem) {
   variant_blocks<TILE, V, 0>(a, smem, w, tr, tc);
 }
 
+
+// ------------------------------------------------------------------ 6. dual
+// Two cells' cores factored in one workgroup, each with the library's pair
+// scheme and buffers of its own, their pair rounds sharing one barrier so
+// that one core's chain overlaps the other's.
+template <int TILE, int B>
+__device__ inline void pair_round(double (&a)[TILE][TILE], int kk, double* lbuf, double* ubuf, double* rcs, int tr,
+                                  int tc, double& rcown) {
+  constexpr int MP = G * TILE;
+  const int k0 = B * G + kk, buf = (k0 >> 1) & 1;
+  const double* c0 = lbuf + 2 * buf * MP;
+  const double* c1 = c0 + MP;
+  const double* r0 = ubuf + 2 * buf * MP;
+  const double* r1 = r0 + MP;
+  const double rc0 = rcs[k0];
+  const double l10 = c0[B * G + kk + 1] * rc0, u01 = r0[B * G + kk + 1];
+  const double rc1 = pivot_rcp(fma(-l10, u01, r1[B * G + kk + 1]));
+  rcown = (tc == kk) ? rc0 : ((tc == kk + 1) ? rc1 : rcown);
+  double u0[TILE], u1[TILE];
+#pragma unroll
+  for (int tj = B; tj < TILE; ++tj) {
+    u0[tj] = r0[tj * G + tc];
+    u1[tj] = fma(-l10, u0[tj], r1[tj * G + tc]);
+  }
+  u0[B] = (tc > kk) ? u0[B] : 0.0;
+  u1[B] = (tc > kk + 1) ? u1[B] : 0.0;
+#pragma unroll
+  for (int ti = B; ti < TILE; ++ti) {
+    double l0 = c0[ti * G + tr] * rc0;
+    if (ti == B) l0 = (tr > kk) ? l0 : 0.0;
+    double l1 = fma(-l0, u01, c1[ti * G + tr]) * rc1;
+    if (ti == B) l1 = (tr > kk + 1) ? l1 : 0.0;
+#pragma unroll
+    for (int tj = B; tj < TILE; ++tj) a[ti][tj] = fma(-l1, u1[tj], fma(-l0, u0[tj], a[ti][tj]));
+  }
+  if (kk + 2 < G) {
+    publish_pair<TILE, B>(a, kk + 2, buf ^ 1, lbuf, ubuf, rcs, tr, tc);
+  } else if constexpr (B + 1 < TILE) {
+    publish_pair<TILE, B + 1>(a, 0, buf ^ 1, lbuf, ubuf, rcs, tr, tc);
+  }
+}
+
+template <int TILE, int B>
+__device__ inline void dual_blocks(double (&a)[TILE][TILE], double (&b)[TILE][TILE], double* sa, double* sb, int tr,
+                                   int tc) {
+  if constexpr (B < TILE) {
+    constexpr int MP = G * TILE;
+    double rca = 0.0, rcb = 0.0;
+#pragma nounroll
+    for (int kk = 0; kk < G; kk += 2) {
+      pair_round<TILE, B>(a, kk, sa, sa + 4 * MP, sa + 8 * MP, tr, tc, rca);
+      pair_round<TILE, B>(b, kk, sb, sb + 4 * MP, sb + 8 * MP, tr, tc, rcb);
+      __syncthreads();
+    }
+    scale_column<TILE, B>(a, rca, tr, tc);
+    scale_column<TILE, B>(b, rcb, tr, tc);
+    dual_blocks<TILE, B + 1>(a, b, sa, sb, tr, tc);
+  }
+}
+
+// Two pair-scheme buffer sets of 9*MP doubles each.
+template <int TILE>
+__device__ inline void factor_dual(double (&a)[TILE][TILE], double (&b)[TILE][TILE], double* smem) {
+  const int tid = threadIdx.x, tr = tid / G, tc = tid % G;
+  constexpr int MP = G * TILE;
+  double* sa = smem;
+  double* sb = smem + 9 * MP;
+  publish_pair<TILE, 0>(a, 0, 0, sa, sa + 4 * MP, sa + 8 * MP, tr, tc);
+  publish_pair<TILE, 0>(b, 0, 0, sb, sb + 4 * MP, sb + 8 * MP, tr, tc);
+  __syncthreads();
+  dual_blocks<TILE, 0>(a, b, sa, sb, tr, tc);
+}
+
 // ------------------------------------------------------------- harness
 template <int TILE>
 __device__ inline void fill(double (&a)[TILE][TILE], int tr, int tc, unsigned seed) {
@@ -535,7 +610,8 @@ __device__ inline void fill(double (&a)[TILE][TILE], int tr, int tc, unsigned se
 }
 
 template <int TILE, int V>
-__global__ __launch_bounds__(NT) void k_factor_probe(int reps, double* out, unsigned long long* cyc) {
+__global__ __launch_bounds__(NT) void k_factor_probe(int reps, double* out, unsigned long long* cyc,
+                                                     unsigned seed_off) {
   extern __shared__ double smem[];
   PlanDev P{};
   P.tile = TILE;
@@ -546,7 +622,7 @@ __global__ __launch_bounds__(NT) void k_factor_probe(int reps, double* out, unsi
   double acc = 0.0;
   unsigned long long total = 0;
   for (int r = 0; r < reps; ++r) {
-    fill<TILE>(a, tr, tc, blockIdx.x * 7919u + r);
+    fill<TILE>(a, tr, tc, blockIdx.x * 7919u + r + seed_off);
     __syncthreads();
     const unsigned long long t0 = __builtin_amdgcn_s_memtime();
     if constexpr (V == 0)
@@ -566,6 +642,44 @@ __global__ __launch_bounds__(NT) void k_factor_probe(int reps, double* out, unsi
     for (int tj = 0; tj < TILE; ++tj) out[((size_t)blockIdx.x * TILE * TILE + ti * TILE + tj) * NT + tid] = a[ti][tj];
   if (tid == 0) cyc[blockIdx.x] = total;
   if (acc == 12345.678) out[0] = acc;   // keeps the factorizations live
+}
+
+
+// Two cores per workgroup and repetition: a (seed as variant 0's) and b
+// (seed + DUAL_OFF); both final tiles are written, a then b.
+constexpr unsigned DUAL_OFF = 1u << 20;
+
+template <int TILE>
+__global__ __launch_bounds__(NT) void k_factor_dual(int reps, double* out, unsigned long long* cyc) {
+  extern __shared__ double smem[];
+  const int tid = threadIdx.x, tr = tid / G, tc = tid % G;
+  double a[TILE][TILE], b[TILE][TILE];
+  double acc = 0.0;
+  unsigned long long total = 0;
+  for (int r = 0; r < reps; ++r) {
+    fill<TILE>(a, tr, tc, blockIdx.x * 7919u + r);
+    fill<TILE>(b, tr, tc, blockIdx.x * 7919u + r + DUAL_OFF);
+    __syncthreads();
+    const unsigned long long t0 = __builtin_amdgcn_s_memtime();
+    factor_dual<TILE>(a, b, smem);
+    __syncthreads();
+    total += __builtin_amdgcn_s_memtime() - t0;
+#pragma unroll
+    for (int ti = 0; ti < TILE; ++ti)
+#pragma unroll
+      for (int tj = 0; tj < TILE; ++tj) acc += a[ti][tj] + b[ti][tj];
+  }
+  const size_t half = (size_t)gridDim.x * TILE * TILE * NT;
+#pragma unroll
+  for (int ti = 0; ti < TILE; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < TILE; ++tj) {
+      const size_t o = ((size_t)blockIdx.x * TILE * TILE + ti * TILE + tj) * NT + tid;
+      out[o] = a[ti][tj];
+      out[half + o] = b[ti][tj];
+    }
+  if (tid == 0) cyc[blockIdx.x] = total;
+  if (acc == 12345.678) out[0] = acc;
 }
 
 // Latency floors of one pivot-pair round, 72 rounds per repetition.
@@ -619,7 +733,7 @@ struct Result {
 };
 
 template <int TILE, int V>
-Result run_factor(int cus, int per_cu, int reps, size_t lds, std::vector<double>& host) {
+Result run_factor(int cus, int per_cu, int reps, size_t lds, std::vector<double>& host, unsigned seed_off = 0) {
   const int blocks = cus * per_cu;
   const size_t nout = (size_t)blocks * TILE * TILE * NT;
   double* out;
@@ -627,14 +741,14 @@ Result run_factor(int cus, int per_cu, int reps, size_t lds, std::vector<double>
   CK(hipMalloc(&out, nout * sizeof(double)));
   CK(hipMalloc(&cyc, blocks * sizeof(unsigned long long)));
   CK(hipFuncSetAttribute((const void*)k_factor_probe<TILE, V>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL((k_factor_probe<TILE, V>), dim3(blocks), dim3(NT), lds, 0, 1, out, cyc);
+  hipLaunchKernelGGL((k_factor_probe<TILE, V>), dim3(blocks), dim3(NT), lds, 0, 1, out, cyc, seed_off);
   CK(hipGetLastError());
   CK(hipDeviceSynchronize());
   hipEvent_t e0, e1;
   CK(hipEventCreate(&e0));
   CK(hipEventCreate(&e1));
   CK(hipEventRecord(e0));
-  hipLaunchKernelGGL((k_factor_probe<TILE, V>), dim3(blocks), dim3(NT), lds, 0, reps, out, cyc);
+  hipLaunchKernelGGL((k_factor_probe<TILE, V>), dim3(blocks), dim3(NT), lds, 0, reps, out, cyc, seed_off);
   CK(hipGetLastError());
   CK(hipEventRecord(e1));
   CK(hipDeviceSynchronize());
@@ -688,6 +802,49 @@ void variant_line(int cus, int reps, const std::vector<double>& ref, const char*
   fflush(stdout);
 }
 
+
+// Two cores per workgroup (variant 6): time per factorization, and both
+// cores checked against variant 0 on the same matrices.
+template <int TILE>
+void dual_line(int cus, int reps, const std::vector<double>& ref) {
+  const size_t lds = 130 * 1024;
+  const size_t half = (size_t)cus * TILE * TILE * NT;
+  double* out;
+  unsigned long long* cyc;
+  CK(hipMalloc(&out, 2 * half * sizeof(double)));
+  CK(hipMalloc(&cyc, cus * sizeof(unsigned long long)));
+  CK(hipFuncSetAttribute((const void*)k_factor_dual<TILE>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL((k_factor_dual<TILE>), dim3(cus), dim3(NT), lds, 0, 1, out, cyc);
+  CK(hipGetLastError());
+  CK(hipDeviceSynchronize());
+  hipEvent_t e0, e1;
+  CK(hipEventCreate(&e0));
+  CK(hipEventCreate(&e1));
+  CK(hipEventRecord(e0));
+  hipLaunchKernelGGL((k_factor_dual<TILE>), dim3(cus), dim3(NT), lds, 0, reps, out, cyc);
+  CK(hipGetLastError());
+  CK(hipEventRecord(e1));
+  CK(hipDeviceSynchronize());
+  float ms = 0;
+  CK(hipEventElapsedTime(&ms, e0, e1));
+  std::vector<double> h(2 * half);
+  CK(hipMemcpy(h.data(), out, 2 * half * sizeof(double), hipMemcpyDeviceToHost));
+  std::vector<unsigned long long> c(cus);
+  CK(hipMemcpy(c.data(), cyc, cus * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  CK(hipFree(out));
+  CK(hipFree(cyc));
+  CK(hipEventDestroy(e0));
+  CK(hipEventDestroy(e1));
+  std::vector<double> refb;
+  run_factor<TILE, 0>(cus, 1, reps, lds, refb, DUAL_OFF);
+  const std::vector<double> ha(h.begin(), h.begin() + half), hb(h.begin() + half, h.end());
+  double t = 0;
+  for (auto x : c) t += (double)x;
+  printf("| %d | 6 dual (two cores, shared barriers) | %.2f | %.0f | %zu + %zu of %zu |\n", TILE, 1e3 * ms / reps / 2,
+         t / cus / reps / 2, differ(ref, ha), differ(refb, hb), 2 * half);
+  fflush(stdout);
+}
+
 template <int TILE>
 void tile_table(int cus, int reps) {
   std::vector<double> ref;
@@ -699,6 +856,7 @@ void tile_table(int cus, int reps) {
   variant_line<TILE, 3>(cus, reps, ref, "publish-first");
   variant_line<TILE, 4>(cus, reps, ref, "blocked");
   variant_line<TILE, 5>(cus, reps, ref, "quad");
+  dual_line<TILE>(cus, reps, ref);
 }
 
 template <int TILE>
