@@ -235,16 +235,54 @@ __global__ __launch_bounds__(256) void k_norm_control(fode_ctrl_params p, fode_g
                                                       int n, double* Y, const double* Ynew,
                                                       const double* E, const double* atol,
                                                       double rtol, const double* mass) {
-  __shared__ double red[4];
+  // The group's sum, cell by cell, each cell's share formed as cell_err_sum
+  // forms it (thread t's entries t, t + NT, ..., its wave's xor-shuffle tree,
+  // then the wave sums in order), but without its barriers: every wave keeps
+  // its sum of each cell of a chunk in part, and thread 0 adds them up in the
+  // same order.  Same bits as the batched kernel's cell_err_sum.
+  constexpr int CHUNK = 64, WAVES = NT / 64;
+  __shared__ double part[CHUNK * WAVES];
   __shared__ int flags[2];
   const long N = (long)C * n;
   double sum = 0.0;
   const bool step_rows = step_change_rows(p);
-  for (int c = 0; c < C; ++c) {
-    const size_t o = (size_t)c * n;
-    const double cs = step_rows ? cell_err_sum_dae(Y + o, Ynew + o, E + o, atol, rtol, n, red, mass)
-                                : cell_err_sum(Y + o, Ynew + o, E + o, atol, rtol, n, red);
-    sum = sum + cs;
+  constexpr int BATCH = 8;   // cells whose loads are in flight together
+  for (int c0 = 0; c0 < C; c0 += CHUNK) {
+    const int ce = min(C, c0 + CHUNK);
+    for (int cb = c0; cb < ce; cb += BATCH) {
+      double acc[BATCH];
+#pragma unroll
+      for (int u = 0; u < BATCH; ++u) acc[u] = 0.0;
+      for (int i = threadIdx.x; i < n; i += NT) {
+        const bool alg = step_rows && mass[i] == 0.0;
+        const double sc = atol[i];
+#pragma unroll
+        for (int u = 0; u < BATCH; ++u) {
+          if (cb + u < ce) {
+            const size_t o = (size_t)(cb + u) * n + i;
+            const double y = Y[o], yn = Ynew[o];
+            const double e = alg ? yn - y : E[o];
+            const double s = e / (sc + rtol * fmax(fabs(y), fabs(yn)));
+            acc[u] = acc[u] + s * s;
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < BATCH; ++u) {
+        double x = acc[u];
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+        if (cb + u < ce && (threadIdx.x & 63) == 0) part[(cb + u - c0) * WAVES + (threadIdx.x >> 6)] = x;
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int c = c0; c < ce; ++c) {
+        double s = 0.0;
+        for (int w = 0; w < WAVES; ++w) s += part[(c - c0) * WAVES + w];
+        sum = sum + s;
+      }
+    }
+    __syncthreads();
   }
   if (threadIdx.x == 0) {
     const double err = err_from_sum(sum, (double)N);
@@ -256,7 +294,17 @@ __global__ __launch_bounds__(256) void k_norm_control(fode_ctrl_params p, fode_g
   }
   __syncthreads();
   if (flags[0]) {
-    for (long i = threadIdx.x; i < N; i += blockDim.x) Y[i] = Ynew[i];
+    // eight loads in flight before their stores (Y and Ynew are distinct arrays)
+    const long step = blockDim.x;
+    long i = threadIdx.x;
+    for (; i + 7 * step < N; i += 8 * step) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = Ynew[i + u * step];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) Y[i + u * step] = v[u];
+    }
+    for (; i < N; i += step) Y[i] = Ynew[i];
   }
   if (flags[1] == FODE_FINISH) {
     __syncthreads();
