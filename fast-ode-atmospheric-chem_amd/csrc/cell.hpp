@@ -89,7 +89,7 @@ struct JLds {
 };
 
 // The cell's rates, forcing and Jacobian, with the same arithmetic (and
-// bits) as rate/forcing/drate/JFromD, but several items per thread per
+// bits) as rate/forcing/drate, but several items per thread per
 // round so that their table loads (global) and LDS gathers are in flight
 // together instead of one dependent chain after another.  Two-reactant-slot
 // mechanisms (P == 2) take straight-line paths; others the generic ones.
@@ -121,20 +121,7 @@ __device__ inline void forcing_into(const MechDev& M, const double* y, const dou
       out[i] = forcing(M, y, rates, i);
       continue;
     }
-    double f = 0.0;
-    int q = M.rhs_ptr[i];
-    const int qe = M.rhs_ptr[i + 1];
-    for (; q + 4 <= qe; q += 4) {
-      const int x0 = M.rhs_rxn[q], x1 = M.rhs_rxn[q + 1], x2 = M.rhs_rxn[q + 2], x3 = M.rhs_rxn[q + 3];
-      const double c0 = M.rhs_coef[q], c1 = M.rhs_coef[q + 1], c2 = M.rhs_coef[q + 2], c3 = M.rhs_coef[q + 3];
-      const double r0 = rates[x0], r1 = rates[x1], r2 = rates[x2], r3 = rates[x3];
-      f = f + r0 * c0;
-      f = f + r1 * c1;
-      f = f + r2 * c2;
-      f = f + r3 * c3;
-    }
-    for (; q < qe; ++q) f = f + rates[M.rhs_rxn[q]] * M.rhs_coef[q];
-    out[i] = f;
+    out[i] = ordered_dot(0.0, rates, M.rhs_rxn, M.rhs_coef, M.rhs_ptr[i], M.rhs_ptr[i + 1]);
   }
   __syncthreads();
 }
@@ -182,7 +169,7 @@ __device__ inline void jacobian_into(const MechDev& M, const CellLds& V) {
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      for (int q = qa[u] + 1; q < qb[u]; ++q) v[u] = v[u] + M.jac_coef[q] * V.rd[M.jac_src[q]];
+      if (qa[u] + 1 < qb[u]) v[u] = ordered_dot(v[u], V.rd, M.jac_src, M.jac_coef, qa[u] + 1, qb[u]);
       if (e0 + u * NT < M.nnz) V.jv[e0 + u * NT] = v[u] + c[u];
     }
   }

@@ -30,6 +30,31 @@ __device__ inline double drate(const MechDev& M, const double* y, const double* 
   return prod;
 }
 
+// f + sum_q vals[src[q]] * coef[q] over q in [q, qe), added in q order (the
+// same bits as one term at a time), with the table loads and gathers of TERMS
+// terms in flight together: a long row (TS1 has species with 192 terms and a
+// Jacobian entry with 110) costs one load latency per TERMS terms, not per
+// term.
+constexpr int TERMS = 8;
+__device__ inline double ordered_dot(double f, const double* vals, const int* src, const double* coef, int q,
+                                     int qe) {
+  for (; q + TERMS <= qe; q += TERMS) {
+    int x[TERMS];
+    double c[TERMS], r[TERMS];
+#pragma unroll
+    for (int u = 0; u < TERMS; ++u) {
+      x[u] = src[q + u];
+      c[u] = coef[q + u];
+    }
+#pragma unroll
+    for (int u = 0; u < TERMS; ++u) r[u] = vals[x[u]];
+#pragma unroll
+    for (int u = 0; u < TERMS; ++u) f = f + r[u] * c[u];
+  }
+  for (; q < qe; ++q) f = f + vals[src[q]] * coef[q];
+  return f;
+}
+
 // Forcing of species i given the rates (or, for a constraint row,
 // sum_s c_s y_s - total).
 __device__ inline double forcing(const MechDev& M, const double* y, const double* rates, int i) {
@@ -40,21 +65,7 @@ __device__ inline double forcing(const MechDev& M, const double* y, const double
     for (int s = 0; s < M.n; ++s) g = g + cf[s] * y[s];
     return g - M.con_total[ci];
   }
-  double f = 0.0;
-  for (int q = M.rhs_ptr[i]; q < M.rhs_ptr[i + 1]; ++q) f = f + rates[M.rhs_rxn[q]] * M.rhs_coef[q];
-  return f;
+  return ordered_dot(0.0, rates, M.rhs_rxn, M.rhs_coef, M.rhs_ptr[i], M.rhs_ptr[i + 1]);
 }
-
-// Jacobian value of entry e from the rate partials d (R*P) in LDS, the same
-// arithmetic as the standalone Jacobian kernel.
-struct JFromD {
-  MechDev M;
-  const double* d;
-  __device__ double operator()(int e) const {
-    double v = 0.0;
-    for (int q = M.jac_ptr[e]; q < M.jac_ptr[e + 1]; ++q) v = v + M.jac_coef[q] * d[M.jac_src[q]];
-    return v + M.jac_const[e];
-  }
-};
 
 }  // namespace fode

@@ -82,9 +82,21 @@ __global__ void k_rhs(MechDev M, int C, const double* __restrict__ Y, const doub
       for (int s = 0; s < M.n; ++s) f = f + cf[s] * y[s];
       f = f - M.con_total[ci];
     } else {
+      // in term order, TERMS terms' rates in flight together
       f = 0.0;
-      for (int q = M.rhs_ptr[i]; q < M.rhs_ptr[i + 1]; ++q)
-        f = f + rate(M, y, kc, M.rhs_rxn[q]) * M.rhs_coef[q];
+      int q = M.rhs_ptr[i];
+      const int qe = M.rhs_ptr[i + 1];
+      for (; q + TERMS <= qe; q += TERMS) {
+        double r[TERMS], cf[TERMS];
+#pragma unroll
+        for (int u = 0; u < TERMS; ++u) {
+          r[u] = rate(M, y, kc, M.rhs_rxn[q + u]);
+          cf[u] = M.rhs_coef[q + u];
+        }
+#pragma unroll
+        for (int u = 0; u < TERMS; ++u) f = f + r[u] * cf[u];
+      }
+      for (; q < qe; ++q) f = f + rate(M, y, kc, M.rhs_rxn[q]) * M.rhs_coef[q];
     }
     F[idx] = f;
   }
@@ -98,8 +110,22 @@ __global__ void k_jac(MechDev M, int C, const double* __restrict__ Y, const doub
     const int c = (int)(idx / M.nnz), e = (int)(idx % M.nnz);
     const double* y = Y + (size_t)c * M.n;
     const double* kc = k + (size_t)c * M.R;
+    // in term order, TERMS terms' partials in flight together
     double v = 0.0;
-    for (int q = M.jac_ptr[e]; q < M.jac_ptr[e + 1]; ++q) {
+    int q = M.jac_ptr[e];
+    const int qe = M.jac_ptr[e + 1];
+    for (; q + TERMS <= qe; q += TERMS) {
+      double d[TERMS], cf[TERMS];
+#pragma unroll
+      for (int u = 0; u < TERMS; ++u) {
+        const int jp = M.jac_src[q + u];
+        d[u] = drate(M, y, kc, jp / M.P, jp % M.P);
+        cf[u] = M.jac_coef[q + u];
+      }
+#pragma unroll
+      for (int u = 0; u < TERMS; ++u) v = v + cf[u] * d[u];
+    }
+    for (; q < qe; ++q) {
       const int jp = M.jac_src[q];
       v = v + M.jac_coef[q] * drate(M, y, kc, jp / M.P, jp % M.P);
     }
