@@ -10,7 +10,7 @@
 #include <string>
 #include <vector>
 
-#include "engine.hpp"
+#include "cell.hpp"
 
 using namespace fode;
 
@@ -131,6 +131,42 @@ __global__ void k_jac(MechDev M, int C, const double* __restrict__ Y, const doub
     }
     J[idx] = v + M.jac_const[e];
   }
+}
+
+// One workgroup per cell: the cell's state and its R rates (k_rhs_cell) or
+// R*P rate partials (k_jac_cell) in LDS, each computed once, then every
+// forcing row or Jacobian entry summed from LDS in term order.  These are
+// the batched kernel's rates_into / forcing_into / jacobian_into, so the
+// bits are those of k_rhs / k_jac (the same products, added in the same
+// order), without a long row's terms each recomputing their rate from
+// global tables.  The host uses them when the LDS fits (cell_lds_bytes);
+// k_rhs / k_jac serve larger mechanisms.
+__global__ __launch_bounds__(NT) void k_rhs_cell(MechDev M, const double* __restrict__ Y,
+                                                 const double* __restrict__ k, double* __restrict__ F) {
+  extern __shared__ double smem[];
+  const int c = blockIdx.x;
+  double* y = smem;
+  double* rates = smem + M.n;
+  const double* yc = Y + (size_t)c * M.n;
+  for (int i = threadIdx.x; i < M.n; i += NT) y[i] = yc[i];
+  __syncthreads();
+  rates_into(M, y, k + (size_t)c * M.R, rates);
+  forcing_into(M, y, rates, F + (size_t)c * M.n);
+}
+
+__global__ __launch_bounds__(NT) void k_jac_cell(MechDev M, const double* __restrict__ Y,
+                                                 const double* __restrict__ k, double* __restrict__ J) {
+  extern __shared__ double smem[];
+  const int c = blockIdx.x;
+  CellLds V{};
+  V.y = smem;
+  V.rd = smem + M.n;
+  V.kc = const_cast<double*>(k + (size_t)c * M.R);   // read only
+  V.jv = J + (size_t)c * M.nnz;
+  const double* yc = Y + (size_t)c * M.n;
+  for (int i = threadIdx.x; i < M.n; i += NT) V.y[i] = yc[i];
+  __syncthreads();
+  jacobian_into(M, V);
 }
 
 struct JGlobal {
@@ -440,6 +476,13 @@ int grid_for(long N, int block = 256) {
   return (int)g;
 }
 
+// LDS of k_rhs_cell / k_jac_cell: the state and one value per rate (or
+// rate partial), plus one, which an entry without terms reads (its product
+// is discarded) when the mechanism has no reactions; within the 64 KB a
+// workgroup gets without opting in.
+constexpr size_t CELL_LDS_MAX = 65536;
+size_t cell_lds_bytes(int n, int values) { return ((size_t)n + (size_t)values + 1) * sizeof(double); }
+
 size_t engine_bytes(const PlanDev& P) {
   return (size_t)engine_lds_doubles(P) * sizeof(double);
 }
@@ -633,7 +676,11 @@ int fode_rhs(void* mech, int C, const double* Y, const double* k, double* F, voi
   if (!mech || C < 0) return fail("fode_rhs: bad argument");
   if (C == 0) return 0;
   const MechDev& M = static_cast<MechHandle*>(mech)->d;
-  hipLaunchKernelGGL(k_rhs, dim3(grid_for((long)C * M.n)), dim3(256), 0, S(stream), M, C, Y, k, F);
+  const size_t bytes = cell_lds_bytes(M.n, M.R);
+  if (bytes <= CELL_LDS_MAX)
+    hipLaunchKernelGGL(k_rhs_cell, dim3(C), dim3(NT), bytes, S(stream), M, Y, k, F);
+  else
+    hipLaunchKernelGGL(k_rhs, dim3(grid_for((long)C * M.n)), dim3(256), 0, S(stream), M, C, Y, k, F);
   HIP_OK(hipGetLastError());
   return 0;
 }
@@ -642,7 +689,11 @@ int fode_jac(void* mech, int C, const double* Y, const double* k, double* J, voi
   if (!mech || C < 0) return fail("fode_jac: bad argument");
   if (C == 0) return 0;
   const MechDev& M = static_cast<MechHandle*>(mech)->d;
-  hipLaunchKernelGGL(k_jac, dim3(grid_for((long)C * M.nnz)), dim3(256), 0, S(stream), M, C, Y, k, J);
+  const size_t bytes = cell_lds_bytes(M.n, M.R * M.P);
+  if (bytes <= CELL_LDS_MAX)
+    hipLaunchKernelGGL(k_jac_cell, dim3(C), dim3(NT), bytes, S(stream), M, Y, k, J);
+  else
+    hipLaunchKernelGGL(k_jac, dim3(grid_for((long)C * M.nnz)), dim3(256), 0, S(stream), M, C, Y, k, J);
   HIP_OK(hipGetLastError());
   return 0;
 }

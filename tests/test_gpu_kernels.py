@@ -54,6 +54,31 @@ def test_rhs_and_jacobian_match_oracle(which):
     assert np.allclose(J, Jo, rtol=1e-13, atol=0.0)
 
 
+def test_rhs_and_jacobian_of_a_mechanism_beyond_the_cell_lds():
+    """40 species and 8,200 reactions: the rates and the rate partials exceed
+    the 64 KB a per-cell workgroup holds, so the per-row and per-entry
+    kernels evaluate (rows of about 800 terms)."""
+    from fastode_amd.mechanism import log_uniform_state
+    from fastode_amd.system import ChemSystem
+    syn = synthetic(40, 8200, 3)
+    assert (40 + 8200 + 1) * 8 > 65536
+    rng = np.random.default_rng(3)
+    C = 6
+    Y = log_uniform_state(C, 40, rng)
+    k = syn.k * rng.uniform(0.5, 2.0, size=(C, 8200))
+    system = ChemSystem(syn.mech, k)
+    om = oracle_mech(syn.mech)
+    F = system.rhs(dev(Y)).cpu().numpy()
+    Fo = om.rhs(k, Y)
+    # a row's rounding is bounded by a multiple of eps times the sum of its
+    # terms' magnitudes (rates are positive here)
+    terms = (k * om._gather(Y).prod(axis=2)) @ np.abs(om.ST)
+    assert (np.abs(F - Fo) <= 1e-12 * terms + 1e-300).all()
+    J = system.jac(dev(Y)).cpu().numpy()
+    Jo = om.jac_values(k, Y)
+    assert np.allclose(J, Jo, rtol=1e-12, atol=0.0)
+
+
 @pytest.mark.parametrize("n,R,seed,alpha", [(20, 40, 1, 1e-2), (100, 250, 4, 1e-3),
                                               (200, 500, 2, 1e-3), (200, 500, 2, 10.0)])
 def test_factor_and_solve_match_numpy(n, R, seed, alpha):
