@@ -95,12 +95,16 @@ def choose_split(F, rho=DENSE_RHO, m_max=G * TILE_MAX):
     return int(max(c, n - m_max))
 
 
-def _stage(levels):
-    """Sort task ids by level; return (order, stage_ptr)."""
+def _stage(levels, key=None):
+    """Sort task ids by level (then by ``key``, stably); return (order,
+    stage_ptr)."""
     levels = np.asarray(levels, dtype=np.int64)
     if len(levels) == 0:
         return np.zeros(0, dtype=np.int64), np.zeros(1, dtype=np.int64)
-    order = np.argsort(levels, kind="stable")
+    if key is None:
+        order = np.argsort(levels, kind="stable")
+    else:
+        order = np.lexsort((np.asarray(key, dtype=np.int64), levels))
     nst = int(levels.max()) + 1
     counts = np.bincount(levels, minlength=nst)
     return order, np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
@@ -285,7 +289,15 @@ def build_plan(n, rows, cols, tile=None, rho=DENSE_RHO, order=None, m_max=None):
     sch_base[NT] = len(sch_words)
     sch_pairs = np.asarray(sch_words, dtype=np.uint32)
 
-    # ---- sparse factorization tasks (dot-product form)
+    # ---- sparse factorization tasks (dot-product form).  An entry's chain
+    # A_ij - L_ip1 U_p1j - L_ip2 U_p2j - ... runs in ascending p, and each
+    # multiply-add goes in the first stage after which both of its inputs are
+    # final (but not before the previous one's): the chain is cut into
+    # consecutive runs, one task per stage, its partial kept in the entry's
+    # LDS slot between them, and the division by U_jj goes with the last run.
+    # The operations and their order are those of one task in the entry's
+    # final stage, so the bits are too; the longest run of a stage, not the
+    # longest chain, sets the stage's time.
     tgt, pstart, pcount, div, level = [], [], [], [], []
     pairs_a, pairs_b = [], []
     lev = np.zeros(ns, dtype=np.int64)          # stage after which an entry is final
@@ -294,26 +306,40 @@ def build_plan(n, rows, cols, tile=None, rho=DENSE_RHO, order=None, m_max=None):
     for e in order_e:
         i, j = int(si[e]), int(sj[e])
         ps = [p for p in range(min(i, j)) if F[i, p] and F[p, j]]
-        L = 0
+        d = sp_index[j, j] if i > j else -1
+        if not ps and d < 0:
+            continue            # the assembled value is final
         start = len(pairs_a)
+        ready, at = [], 0
         for p in ps:
             a, b = sp_index[i, p], sp_index[p, j]
             pairs_a.append(a)
             pairs_b.append(b)
-            L = max(L, lev[a] + 1, lev[b] + 1)
-        d = -1
-        if i > j:
-            d = sp_index[j, j]
-            L = max(L, lev[d] + 1)
-        lev[e] = L
-        if not ps and d < 0:
-            continue            # the assembled value is final
-        tgt.append(e)
-        pstart.append(start)
-        pcount.append(len(ps))
-        div.append(d)
-        level.append(L - 1)
-    fo, fptr = _stage(level)
+            at = max(at, lev[a], lev[b])
+            ready.append(at)
+        final = max(ready[-1] if ready else 0, lev[d] if d >= 0 else 0)
+        q = 0
+        while q < len(ps):
+            stage = ready[q] if ready[q] != ready[-1] else final
+            r = q
+            while r < len(ps) and (ready[r] == ready[q]):
+                r += 1
+            if stage == final:
+                r = len(ps)
+            tgt.append(e)
+            pstart.append(start + q)
+            pcount.append(r - q)
+            div.append(d if r == len(ps) else -1)
+            level.append(stage)
+            q = r
+        if not ps:
+            tgt.append(e)
+            pstart.append(start)
+            pcount.append(0)
+            div.append(d)
+            level.append(final)
+        lev[e] = final + 1
+    fo, fptr = _stage(level, [-c_ for c_ in pcount])
     f_tasks = _pack_tasks(*(np.asarray(x)[fo] if len(tgt) else [] for x in (tgt, pstart, pcount, div)))
     f_pairs = _pack_pairs(pairs_a, pairs_b)
 

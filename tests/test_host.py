@@ -72,6 +72,13 @@ def test_plan_fill_is_a_superset_and_levels_are_causal():
                 w = int(a["f_pairs"][q])
                 for src in (w >> 16, w & 0xFFFF):
                     assert done_stage.get(src, -1) < s
+    # the division source too, and its task is the entry's last
+    for s in range(len(ptr) - 1):
+        for t in range(ptr[s], ptr[s + 1]):
+            d = int(tasks[t, 0] >> 16)
+            if d != 0xFFFF:
+                assert done_stage.get(d, -1) < s
+                assert done_stage[int(tasks[t, 0] & 0xFFFF)] == s
     # Schur sums: each used by exactly one core slot, closed by exactly one
     # pair of its owner's list, whose factors are sparse entries (final after
     # the factor tasks, which the Schur update follows)
@@ -84,6 +91,51 @@ def test_plan_fill_is_a_superset_and_levels_are_causal():
     for t in range(256):
         assert int((pr[base[t]:base[t + 1]] >> 31).sum()) == per_thread[t]
     assert (((pr >> 16) & 0x7FFF) < plan.ns).all() and ((pr & 0xFFFF) < plan.ns).all()
+
+
+@pytest.mark.parametrize("which", ["ts1", "synthetic200"])
+def test_factor_chains_keep_their_order_across_stages(which):
+    """An entry's factor chain may be cut into runs over several stages; the
+    runs, taken in stage order, are exactly its multiply-adds in ascending
+    pivot order (so the bits are those of one task), each stage updates an
+    entry at most once, and the division comes with the last run."""
+    if which == "ts1":
+        from fastode_amd.ts1 import TS1
+        mech = TS1().mech
+    else:
+        mech = synthetic(200, 500, 2).mech
+    plan = build_plan(mech.n, mech.rows, mech.cols)
+    F, c = plan.F, plan.c
+    si, sj = np.nonzero(F)
+    keep = np.minimum(si, sj) < c
+    si, sj = si[keep], sj[keep]
+    index = {(int(i), int(j)): e for e, (i, j) in enumerate(zip(si, sj))}
+    a = plan.arrays
+    tasks = a["f_tasks"].reshape(-1, 2).astype(np.int64)
+    ptr = a["f_ptr"]
+    chain, divs, cut = {}, {}, 0
+    for s in range(len(ptr) - 1):
+        seen = set()
+        for t in range(ptr[s], ptr[s + 1]):
+            e = int(tasks[t, 0] & 0xFFFF)
+            assert e not in seen
+            seen.add(e)
+            assert e not in divs, "a run after the division"
+            start, count = int(tasks[t, 1] & 0xFFFFF), int(tasks[t, 1] >> 20)
+            cut += e in chain
+            chain.setdefault(e, []).extend(int(w) for w in a["f_pairs"][start:start + count])
+            if tasks[t, 0] >> 16 != 0xFFFF:
+                divs[e] = int(tasks[t, 0] >> 16)
+    for e, (i, j) in enumerate(zip(si, sj)):
+        i, j = int(i), int(j)
+        want = [(index[(i, p)] << 16) | index[(p, j)] for p in range(min(i, j)) if F[i, p] and F[p, j]]
+        assert chain.get(e, []) == want
+        if i > j:
+            assert divs[e] == index[(j, j)]
+        else:
+            assert e not in divs
+    if which == "ts1":
+        assert cut > 0          # TS1's long chains are cut
 
 
 def test_mechanism_tables_match_the_oracle():
