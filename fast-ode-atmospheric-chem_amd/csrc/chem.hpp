@@ -51,7 +51,24 @@ __device__ inline double ordered_dot(double f, const double* vals, const int* sr
 #pragma unroll
     for (int u = 0; u < TERMS; ++u) f = f + r[u] * c[u];
   }
-  for (; q < qe; ++q) f = f + vals[src[q]] * coef[q];
+  if (q < qe) {
+    // the last one to TERMS - 1 terms: loads in flight together (indices
+    // clamped into the row, the extra terms discarded), added in order
+    const int last = qe - 1;
+    int x[TERMS - 1];
+    double c[TERMS - 1], r[TERMS - 1];
+#pragma unroll
+    for (int u = 0; u < TERMS - 1; ++u) {
+      const int qq = q + u < last ? q + u : last;
+      x[u] = src[qq];
+      c[u] = coef[qq];
+    }
+#pragma unroll
+    for (int u = 0; u < TERMS - 1; ++u) r[u] = vals[x[u]];
+#pragma unroll
+    for (int u = 0; u < TERMS - 1; ++u)
+      if (q + u < qe) f = f + r[u] * c[u];
+  }
   return f;
 }
 

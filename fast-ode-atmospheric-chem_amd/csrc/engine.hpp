@@ -111,9 +111,18 @@ __device__ inline void task_single(const uint2 tk, const uint32_t* pairs, double
     acc = fma(-x2, y2, acc);
     acc = fma(-x3, y3, acc);
   }
-  for (; q < qe; ++q) {
-    const uint32_t w = pairs[q];
-    acc = fma(-lds[w >> 16], lds[w & 0xFFFFu], acc);
+  if (q < qe) {
+    // the last one to three pairs: their loads in flight together (indices
+    // clamped into the task, the extra products discarded), then applied in
+    // order
+    const uint32_t last = qe - 1;
+    const uint32_t w0 = pairs[q], w1 = pairs[min(q + 1, last)], w2 = pairs[min(q + 2, last)];
+    const double x0 = lds[w0 >> 16], y0 = lds[w0 & 0xFFFFu];
+    const double x1 = lds[w1 >> 16], y1 = lds[w1 & 0xFFFFu];
+    const double x2 = lds[w2 >> 16], y2 = lds[w2 & 0xFFFFu];
+    acc = fma(-x0, y0, acc);
+    if (q + 1 < qe) acc = fma(-x1, y1, acc);
+    if (q + 2 < qe) acc = fma(-x2, y2, acc);
   }
   if (div != NO_DIV) acc = acc * pivot_rcp(lds[div]);
   lds[tgt] = acc;
@@ -147,9 +156,17 @@ __device__ inline void task_quad(const uint2* tasks, int t, int e, const uint32_
       part = fma(x2, y2, part);
       part = fma(x3, y3, part);
     }
-    for (; q < qe; q += 4) {
-      const uint32_t w = pairs[q];
-      part = fma(lds[w >> 16], lds[w & 0xFFFFu], part);
+    if (q < qe) {
+      // this lane's last one to three pairs (q, q + 4, q + 8), loads in
+      // flight together as in task_single
+      const uint32_t last = qe - 1;
+      const uint32_t w0 = pairs[q], w1 = pairs[min(q + 4, last)], w2 = pairs[min(q + 8, last)];
+      const double x0 = lds[w0 >> 16], y0 = lds[w0 & 0xFFFFu];
+      const double x1 = lds[w1 >> 16], y1 = lds[w1 & 0xFFFFu];
+      const double x2 = lds[w2 >> 16], y2 = lds[w2 & 0xFFFFu];
+      part = fma(x0, y0, part);
+      if (q + 4 < qe) part = fma(x1, y1, part);
+      if (q + 8 < qe) part = fma(x2, y2, part);
     }
   }
   part += dpp_mov<0xB1>(part);   // quad_perm [1, 0, 3, 2]
