@@ -206,7 +206,7 @@ __global__ __launch_bounds__(NT) void k_adjoint(AdjArgs A) {
       for (int i = tid; i < n; i += NT) V.y[i] = w[i];
       for (int i = tid; i < S * n; i += NT) V.K[i] = w[n + i];
       __syncthreads();
-      jacobian_into(M, V);
+      jacobian_into<slot3_paths(TILE)>(M, V);
       double a[TILE][TILE];
       assemble<TILE>(A.PT, JLds{V.jv}, alpha, D.mass, false, L.lds, a);
       factor_cell<TILE>(A.PT, A.PT.t, L, a);

@@ -103,8 +103,9 @@ __device__ __forceinline__ void load_cell(const BatchArgs& A, const CellLds& V, 
 
 // The projection's residual at y (LDS): F_i(y) on algebraic rows, 0 on the
 // others (fastode/solvers.py:110-114), into g.
+template <int TILE>
 __device__ __forceinline__ void residual_into(const BatchArgs& A, const CellLds& V, const double* y, double* g) {
-  rates_into(A.M, y, V.kc, V.rd);
+  rates_into<slot3_paths(TILE)>(A.M, y, V.kc, V.rd);
   for (int i = threadIdx.x; i < A.M.n; i += NT) g[i] = A.b.mass[i] == 0.0 ? forcing(A.M, y, V.rd, i) : 0.0;
   __syncthreads();
 }
@@ -116,7 +117,7 @@ __device__ __forceinline__ void alg_solve(const BatchArgs& A, const Tables& T, c
                           const double* rhs, double* out) {
   const PlanDev& P = A.P;
   const int n = A.M.n;
-  jacobian_into(A.M, V);
+  jacobian_into<slot3_paths(TILE)>(A.M, V);
   double a[TILE][TILE];
   assemble<TILE>(P, JLds{V.jv}, 1.0, A.b.mass, true, L.lds, a);
   factor_cell<TILE>(P, T, L, a);
@@ -156,7 +157,7 @@ __device__ __forceinline__ bool project_group(const BatchArgs& A, const Tables& 
     double gmax = 0.0, bad = 0.0;
     for (int c = 0; c < cells; ++c) {
       load_cell(A, V, c0 + c);
-      residual_into(A, V, V.y, V.f0);
+      residual_into<TILE>(A, V, V.y, V.f0);
       for (int i = tid; i < n; i += NT)
         if (mass[i] == 0.0) {
           const double gi = V.f0[i];
@@ -177,7 +178,7 @@ __device__ __forceinline__ bool project_group(const BatchArgs& A, const Tables& 
     bad = 0.0;
     for (int c = 0; c < cells; ++c) {
       load_cell(A, V, c0 + c);
-      residual_into(A, V, V.y, V.f0);
+      residual_into<TILE>(A, V, V.y, V.f0);
       alg_solve<TILE>(A, T, L, V, V.f0, V.fr);
       for (int i = tid; i < n; i += NT)
         if (mass[i] == 0.0) {
@@ -213,7 +214,7 @@ __device__ __forceinline__ bool project_group(const BatchArgs& A, const Tables& 
           V.w[i] = v;
         }
         __syncthreads();
-        residual_into(A, V, V.w, V.f0);
+        residual_into<TILE>(A, V, V.w, V.f0);
         for (int i = tid; i < n; i += NT)
           if (mass[i] == 0.0) {
             if (!isfinite(V.f0[i])) gbad = 1.0;
@@ -238,7 +239,7 @@ __device__ __forceinline__ bool project_group(const BatchArgs& A, const Tables& 
         for (int i = tid; i < n; i += NT)
           V.w[i] = mass[i] == 0.0 ? V.y[i] + used * Dg[(size_t)c * n + i] : V.y[i];
         __syncthreads();
-        residual_into(A, V, V.w, V.f0);
+        residual_into<TILE>(A, V, V.w, V.f0);
         alg_solve<TILE>(A, T, L, V, V.f0, V.fr);     // the factorization at Y, as LU.solve reuses it
         for (int i = tid; i < n; i += NT)
           if (mass[i] == 0.0) {
@@ -408,9 +409,9 @@ __device__ __forceinline__ void be_cell_pass(const BatchArgs& A, const Tables& T
   if (B.k_stride != 0)
     for (int j = tid; j < M.R; j += NT) V.kc[j] = kg[j] * B.kmul[j];
   __syncthreads();
-  rates_into(M, V.y, V.kc, V.rd);
+  rates_into<slot3_paths(TILE)>(M, V.y, V.kc, V.rd);
   forcing_into(M, V.y, V.rd, V.f0);
-  jacobian_into(M, V);
+  jacobian_into<slot3_paths(TILE)>(M, V);
   double a[TILE][TILE];
   assemble<TILE>(P, JLds{V.jv}, g.alpha, B.mass, false, L.lds, a);
   factor_cell<TILE>(P, T, L, a);

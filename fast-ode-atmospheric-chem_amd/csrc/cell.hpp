@@ -92,8 +92,16 @@ struct JLds {
 // bits) as rate/forcing/drate, but several items per thread per
 // round so that their table loads (global) and LDS gathers are in flight
 // together instead of one dependent chain after another.  Two-reactant-slot
-// mechanisms (P == 2) take straight-line paths; others the generic ones.
+// mechanisms (P == 2) and three-slot ones (P == 3) take straight-line paths; others the
+// generic ones.
 
+// Whether a kernel of core tile TILE takes the three-slot paths.  The large
+// cores' kernels leave them out: the code alone shifts their register
+// allocation (measured about 0.5% slower on a two-slot mechanism), and the
+// generic path gives the same bits.
+constexpr bool slot3_paths(int tile) { return tile <= 5; }
+
+template <bool SLOT3 = true>
 __device__ inline void rates_into(const MechDev& M, const double* y, const double* kc, double* rates) {
   if (M.P == 2) {
     for (int j0 = threadIdx.x; j0 < M.R; j0 += 2 * NT) {
@@ -104,6 +112,20 @@ __device__ inline void rates_into(const MechDev& M, const double* y, const doubl
         const int s0 = M.react[2 * jj], s1 = M.react[2 * jj + 1];
         const double v0 = s0 < M.n ? y[s0] : 1.0, v1 = s1 < M.n ? y[s1] : 1.0;
         r[u] = kc[jj] * (v0 * v1);
+      }
+#pragma unroll
+      for (int u = 0; u < 2; ++u)
+        if (j0 + u * NT < M.R) rates[j0 + u * NT] = r[u];
+    }
+  } else if (SLOT3 && M.P == 3) {
+    for (int j0 = threadIdx.x; j0 < M.R; j0 += 2 * NT) {
+      double r[2];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int j = j0 + u * NT, jj = j < M.R ? j : 0;
+        const int s0 = M.react[3 * jj], s1 = M.react[3 * jj + 1], s2 = M.react[3 * jj + 2];
+        const double v0 = s0 < M.n ? y[s0] : 1.0, v1 = s1 < M.n ? y[s1] : 1.0, v2 = s2 < M.n ? y[s2] : 1.0;
+        r[u] = kc[jj] * ((v0 * v1) * v2);
       }
 #pragma unroll
       for (int u = 0; u < 2; ++u)
@@ -127,6 +149,7 @@ __device__ inline void forcing_into(const MechDev& M, const double* y, const dou
 }
 
 // Rate partials at V.y into V.rd, then the Jacobian values into V.jv.
+template <bool SLOT3 = true>
 __device__ inline void jacobian_into(const MechDev& M, const CellLds& V) {
   const int tid = threadIdx.x, RP = M.R * M.P;
   if (M.P == 2) {
@@ -138,6 +161,22 @@ __device__ inline void jacobian_into(const MechDev& M, const CellLds& V) {
         const int jp = jp0 + u * NT, jj = jp < RP ? jp : 0;
         const int s = M.react[jj ^ 1];
         d[u] = V.kc[jj >> 1] * (s < M.n ? V.y[s] : 1.0);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (jp0 + u * NT < RP) V.rd[jp0 + u * NT] = d[u];
+    }
+  } else if (SLOT3 && M.P == 3) {
+    // d rate_j / d y_{react[j,p]} = (k_j * y_{react[j,a]}) * y_{react[j,b]}, a < b the other slots
+    for (int jp0 = tid; jp0 < RP; jp0 += 4 * NT) {
+      double d[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int jp = jp0 + u * NT, jj = jp < RP ? jp : 0;
+        const int j = jj / 3, p = jj - 3 * j;
+        const int sa = M.react[3 * j + (p == 0 ? 1 : 0)], sb = M.react[3 * j + (p == 2 ? 1 : 2)];
+        const double va = sa < M.n ? V.y[sa] : 1.0, vb = sb < M.n ? V.y[sb] : 1.0;
+        d[u] = (V.kc[j] * va) * vb;
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u)
@@ -186,10 +225,10 @@ __device__ __forceinline__ void cell_stages(const PlanDev& P, const Tables& T, c
   const int n = M.n, tid = threadIdx.x;
   // F(Y) (fastode/solvers.py:345) and the Jacobian (:347), recomputed per
   // attempt: they are the same values a retry reuses.
-  rates_into(M, V.y, V.kc, V.rd);
+  rates_into<slot3_paths(TILE)>(M, V.y, V.kc, V.rd);
   forcing_into(M, V.y, V.rd, V.f0);
   FODE_PHASE(PH_RATES);
-  jacobian_into(M, V);
+  jacobian_into<slot3_paths(TILE)>(M, V);
   FODE_PHASE(PH_JACOBIAN);
   double a[TILE][TILE];
   assemble<TILE>(P, JLds{V.jv}, alpha, mass, false, L.lds, a);
@@ -211,7 +250,7 @@ __device__ __forceinline__ void cell_stages(const PlanDev& P, const Tables& T, c
         V.w[i] = v;
       }
       __syncthreads();
-      rates_into(M, V.w, V.kc, V.rd);
+      rates_into<slot3_paths(TILE)>(M, V.w, V.kc, V.rd);
       forcing_into(M, V.w, V.rd, V.fr);
       forcing_src = V.fr;
     }

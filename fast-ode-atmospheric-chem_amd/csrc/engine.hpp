@@ -271,10 +271,35 @@ template <int TILE, class JS>
 __device__ inline void assemble(const PlanDev& P, const JS& jv, double alpha, const double* mass,
                                 bool alg_only, double* lds, double (&a)[TILE][TILE]) {
   const int tid = threadIdx.x, tr = tid / G, tc = tid % G;
-  for (int s = tid; s < P.ns; s += NT) {
-    const int d = P.sp_diag[s];
-    lds[s] = stage_entry(jv, P.sp_src[s], mass[P.sp_row[s]] == 0.0, d >= 0, alpha,
-                         d >= 0 ? mass[d] : 0.0, alg_only);
+  // four sparse entries per thread per round, their table reads and then
+  // their gathers in flight together (entry 0 stands in past the end), for
+  // the small cores, whose plans put thousands of entries in the sparse
+  // part.  The large cores keep one entry per round: there the batched
+  // loads cost registers that the core factor uses (about 1% slower).
+  if constexpr (TILE > 5) {
+    for (int s = tid; s < P.ns; s += NT) {
+      const int d = P.sp_diag[s];
+      lds[s] = stage_entry(jv, P.sp_src[s], mass[P.sp_row[s]] == 0.0, d >= 0, alpha,
+                           d >= 0 ? mass[d] : 0.0, alg_only);
+    }
+  } else for (int s0 = tid; s0 < P.ns; s0 += 4 * NT) {
+    int d[4], e[4], r[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int s = s0 + u * NT, ss = s < P.ns ? s : 0;
+      d[u] = P.sp_diag[ss];
+      e[u] = P.sp_src[ss];
+      r[u] = P.sp_row[ss];
+    }
+    double v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const double mr = mass[r[u]], md = mass[d[u] >= 0 ? d[u] : 0];
+      v[u] = stage_entry_select(jv, e[u], mr == 0.0, d[u] >= 0, alpha, d[u] >= 0 ? md : 0.0, alg_only);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (s0 + u * NT < P.ns) lds[s0 + u * NT] = v[u];
   }
   // the row of each of this thread's core rows: its mass (alg-only mask, diagonal)
   double rmass[TILE];
