@@ -136,6 +136,9 @@ _SIGS = {
     "fode_stage_final": (i32, [P(CtrlParams), i32, i32, vp, P(vp), vp, vp, vp]),
     "fode_norm_control": (i32, [P(CtrlParams), vp, i32, i32, vp, vp, vp, vp, ct.c_double, vp, vp]),
     "fode_clamp": (i32, [i32, i32, vp, vp, i32, vp]),
+    "fode_proj_mask": (i32, [i32, i32, vp, vp, vp, vp]),
+    "fode_proj_candidate": (i32, [i32, i32, vp, vp, ct.c_double, vp, vp, vp]),
+    "fode_proj_measure": (i32, [i32, i32, vp, vp, vp, vp, ct.c_double, vp, vp]),
     "fode_be_residual": (i32, [i32, i32, vp, vp, vp, vp, vp, vp]),
     "fode_scale_columns": (i32, [i32, i32, vp, i32, vp, vp, vp, vp, vp]),
     "fode_be_update": (i32, [P(CtrlParams), vp, i32, i32, vp, vp, vp, vp, ct.c_double, vp]),

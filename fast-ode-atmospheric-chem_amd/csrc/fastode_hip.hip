@@ -469,6 +469,86 @@ __global__ void k_clamp(long N, int n, double* Y, const double* mass, int constr
               (long)gridDim.x * blockDim.x);
 }
 
+// ---- project's arrays (fastode/solvers.py:100-178), plain path.  The Newton
+// decisions stay on the host, one scalar read per decision, as the reference's
+// loop takes them; the arrays they test are formed and reduced here.
+
+// out = where(alg, G, 0): the residual with the differential rows exactly 0
+// (fastode/solvers.py:124).
+__global__ void k_proj_mask(long N, int n, const double* __restrict__ G, const double* __restrict__ mass,
+                            double* __restrict__ out) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x)
+    out[i] = mass[i % n] == 0.0 ? G[i] : 0.0;
+}
+
+// out = Y with Y + step * D on the algebraic rows: a backtracking candidate
+// (fastode/solvers.py:151-153), product then sum, as NumPy rounds them.
+__global__ void k_proj_candidate(long N, int n, const double* __restrict__ Y, const double* __restrict__ D, double step,
+                                 const double* __restrict__ mass, double* __restrict__ out) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+    const double y = Y[i];
+    if (mass[i % n] == 0.0) {
+      const double sd = step * D[i];
+      out[i] = y + sd;
+    } else {
+      out[i] = y;
+    }
+  }
+}
+
+// Over the algebraic rows of V: out[0] = 1 when an entry is not finite, else
+// 0; out[1] = max |V| (Z null: the residual test, fastode/solvers.py:129-132)
+// or the weighted max of project's `weighted(Z, V)` (fastode/solvers.py:112-118):
+// sc = atol + rtol * maximum(|z|, |z + d|), q = |d| / sc where sc > 0, else 0
+// for d == 0 and inf otherwise.  One workgroup; a max does not depend on order.
+constexpr int PROJ_NT = 1024;
+__global__ void __launch_bounds__(PROJ_NT) k_proj_measure(long N, int n, const double* __restrict__ V,
+                                                          const double* __restrict__ Z,
+                                                          const double* __restrict__ mass,
+                                                          const double* __restrict__ atol, double rtol,
+                                                          double* __restrict__ out) {
+  __shared__ double s_bad[PROJ_NT / 64], s_max[PROJ_NT / 64];
+  double bad = 0.0, m = 0.0;
+  for (long i = threadIdx.x; i < N; i += PROJ_NT) {
+    const int col = (int)(i % n);
+    if (mass[col] != 0.0) continue;
+    const double d = V[i];
+    if (!isfinite(d)) {
+      bad = 1.0;
+      continue;
+    }
+    double q;
+    if (Z) {
+      const double z = Z[i], a = fabs(z), b = fabs(z + d);
+      const double big = isnan(a) ? a : (isnan(b) ? b : (a >= b ? a : b));   // np.maximum
+      const double rb = rtol * big;
+      const double sc = atol[col] + rb;
+      q = sc > 0.0 ? fabs(d) / sc : (d == 0.0 ? 0.0 : INFINITY);
+    } else {
+      q = fabs(d);
+    }
+    m = fmax(m, q);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    bad = fmax(bad, __shfl_xor(bad, off));
+    m = fmax(m, __shfl_xor(m, off));
+  }
+  const int w = threadIdx.x / 64;
+  if ((threadIdx.x & 63) == 0) {
+    s_bad[w] = bad;
+    s_max[w] = m;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < PROJ_NT / 64; ++k) {
+      bad = fmax(bad, s_bad[k]);
+      m = fmax(m, s_max[k]);
+    }
+    out[0] = bad;
+    out[1] = m;
+  }
+}
+
 int grid_for(long N, int block = 256) {
   long g = (N + block - 1) / block;
   if (g < 1) g = 1;
@@ -821,6 +901,37 @@ int fode_clamp(int C, int n, double* Y, const double* mass, int constrained, voi
   const long N = (long)C * n;
   if (N == 0) return 0;
   hipLaunchKernelGGL(k_clamp, dim3(grid_for(N)), dim3(256), 0, S(stream), N, n, Y, mass, constrained);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+int fode_proj_mask(int C, int n, const double* G, const double* mass, double* out, void* stream) {
+  if (C < 0 || n <= 0) return fail("fode_proj_mask: bad shape");
+  const long N = (long)C * n;
+  if (N == 0) return 0;
+  if (!G || !mass || !out) return fail("fode_proj_mask: null array");
+  hipLaunchKernelGGL(k_proj_mask, dim3(grid_for(N)), dim3(256), 0, S(stream), N, n, G, mass, out);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+int fode_proj_candidate(int C, int n, const double* Y, const double* D, double step, const double* mass, double* out,
+                        void* stream) {
+  if (C < 0 || n <= 0) return fail("fode_proj_candidate: bad shape");
+  const long N = (long)C * n;
+  if (N == 0) return 0;
+  if (!Y || !D || !mass || !out) return fail("fode_proj_candidate: null array");
+  hipLaunchKernelGGL(k_proj_candidate, dim3(grid_for(N)), dim3(256), 0, S(stream), N, n, Y, D, step, mass, out);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+int fode_proj_measure(int C, int n, const double* V, const double* Z, const double* mass, const double* atol,
+                      double rtol, double* out, void* stream) {
+  if (C < 0 || n <= 0) return fail("fode_proj_measure: bad shape");
+  if (!V || !mass || !out || (Z && !atol)) return fail("fode_proj_measure: null array");
+  hipLaunchKernelGGL(k_proj_measure, dim3(1), dim3(PROJ_NT), 0, S(stream), (long)C * n, n, V, Z, mass, atol, rtol,
+                     out);
   HIP_OK(hipGetLastError());
   return 0;
 }

@@ -133,46 +133,59 @@ def project(system, Y, atol, rtol, counts: Counts, max_updates=10, tol=0.1,
             max_backtracks=24, backtrack_factor=0.5, sigma=1.0e-4):
     """Consistent initialization by damped Newton on the algebraic rows
     (fastode/solvers.py:100-178), on device tensors.  Returns (Y, status);
-    a failure restores Y."""
-    alg_np = system.mass_np == 0.0
-    if not np.any(alg_np):
+    a failure restores Y.
+
+    The residual masking, the backtracking candidates and the finite /
+    zero / weighted-max measures run as HIP kernels (``fode_proj_*``); the
+    host takes the reference's decisions, reading two doubles per measure."""
+    if not np.any(system.mass_np == 0.0):
         return Y, "converged"
-    dev = Y.device
-    alg = torch.as_tensor(alg_np, device=dev)
-    atol_a = as_device(atol)[alg]
+    C, n = Y.shape
+    s = L.stream_handle()
+    mass = system.mass_dev
+    atol_d = as_device(atol, (n,))
+    rtol = float(rtol)
+    meas = torch.empty(2, dtype=F64, device=Y.device)
+
+    def measure(V, Z=None):
+        """(finite, max): max |V| with Z None, else the weighted max of (Z, V)."""
+        L.check(L.lib.fode_proj_measure(C, n, L.ptr(V), L.ptr(Z), L.ptr(mass), L.ptr(atol_d), rtol,
+                                        L.ptr(meas), s), "fode_proj_measure")
+        bad, m = meas.tolist()
+        return bad == 0.0, m
 
     def residual(Z):
-        G = system.rhs(Z)
+        G = system.rhs(Z).contiguous()
         counts.rhs += 1
-        return torch.where(alg, G, torch.zeros_like(G))   # differential rows exactly 0
+        out = torch.empty_like(G)                  # differential rows exactly 0
+        L.check(L.lib.fode_proj_mask(C, n, L.ptr(G), L.ptr(mass), L.ptr(out), s), "fode_proj_mask")
+        return out
 
-    def finite(A):
-        return bool(torch.isfinite(A[:, alg]).all())
+    def candidate(Z, D, step):
+        out = torch.empty_like(Z)
+        L.check(L.lib.fode_proj_candidate(C, n, L.ptr(Z), L.ptr(D), float(step), L.ptr(mass), L.ptr(out), s),
+                "fode_proj_candidate")
+        return out
 
-    def weighted(Z, D):
-        z, d = Z[:, alg], D[:, alg]
-        sc = atol_a[None, :] + rtol * torch.maximum(z.abs(), (z + d).abs())
-        q = torch.where(sc > 0.0, d.abs() / sc,
-                        torch.where(d == 0.0, torch.zeros_like(d), torch.full_like(d, math.inf)))
-        return float(q.max())
-
+    Y = Y.contiguous()
     saved = None
     for update in range(max_updates + 1):
         Gv = residual(Y)
-        if not finite(Gv):
+        ok, gmax = measure(Gv)
+        if not ok:
             break
         counts.proj_passes += 1
-        if float(Gv[:, alg].abs().max()) == 0.0:
+        if gmax == 0.0:
             return Y, "converged"
         J = system.jac(Y)
         counts.jac += 1
         LU = system.factor(1.0, J, algebraic_rows_only=True)
         counts.lu += 1
-        delta = LU.solve(Gv)
+        delta = LU.solve(Gv).contiguous()
         counts.solves += 1
-        if not finite(delta):
+        ok, q = measure(delta, Y)
+        if not ok:
             break
-        q = weighted(Y, delta)
         if q <= tol:
             return Y, "converged"
         if update == max_updates:
@@ -182,24 +195,24 @@ def project(system, Y, atol, rtol, counts: Counts, max_updates=10, tol=0.1,
         step = 1.0
         taken = None
         for _ in range(max_backtracks + 1):
-            cand = Y.clone()
-            cand[:, alg] += step * delta[:, alg]
+            cand = candidate(Y, delta, step)
             used = step
             step *= backtrack_factor
-            if not finite(cand):
+            if not measure(cand)[0]:
                 continue
             Gc = residual(cand)
             counts.proj_trials += 1
-            if not finite(Gc):
+            ok, gcmax = measure(Gc)
+            if not ok:
                 continue
-            if float(Gc[:, alg].abs().max()) == 0.0:
+            if gcmax == 0.0:
                 taken = cand
                 break
-            trial = LU.solve(Gc)
+            trial = LU.solve(Gc).contiguous()
             counts.solves += 1
-            if not finite(trial):
+            ok, qt = measure(trial, cand)
+            if not ok:
                 continue
-            qt = weighted(cand, trial)
             if qt <= tol or qt < (1.0 - sigma * used) * q:
                 taken = cand
                 break

@@ -205,6 +205,25 @@ int fode_scale_columns(int C, int ldk, double* k, int ncols, const int32_t* cols
 /* clamp(system, Y) alone (fastode/solvers.py:89-97). */
 int fode_clamp(int C, int n, double* Y, const double* mass, int constrained, void* stream);
 
+/* ---- project (fastode/solvers.py:100-178), plain path: the arrays of the
+ * damped Newton on the algebraic rows (mass == 0).  The host keeps the
+ * reference's decisions and reads one measure per decision. */
+
+/* out = where(algebraic, G, 0): the residual (fastode/solvers.py:121-124). */
+int fode_proj_mask(int C, int n, const double* G, const double* mass, double* out, void* stream);
+
+/* out = Y, plus step * D on the algebraic rows: a backtracking candidate
+ * (fastode/solvers.py:151-153). */
+int fode_proj_candidate(int C, int n, const double* Y, const double* D, double step, const double* mass, double* out,
+                        void* stream);
+
+/* Over the algebraic rows of V, into out[2] (device): out[0] = 1 if an entry
+ * is not finite (fastode/solvers.py:126-127), out[1] = max |V| when Z is null
+ * (the zero-residual test, :131-132), else the weighted max of
+ * `weighted(Z, V)` (:112-118) with per-species atol and scalar rtol. */
+int fode_proj_measure(int C, int n, const double* V, const double* Z, const double* mass, const double* atol,
+                      double rtol, double* out, void* stream);
+
 /* ---- Batched persistent form: every group of a batch over one host
  * interval in one launch.  It replaces the reference's group loop around
  * Algorithm.solve (fastode/ts1.py:196-216) with identical per-group

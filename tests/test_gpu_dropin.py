@@ -257,3 +257,61 @@ def test_oracle_and_plain_agree_on_a_constrained_dae():
         assert getattr(c, f) == getattr(oc, f), f
     assert c.steps < 250
     assert np.allclose(Y, Yo, rtol=1e-9, atol=1e-17)
+
+
+@pytest.mark.parametrize("case", ["backtracks", "no_root", "already_consistent"])
+def test_project_newton_and_backtracking_match_the_oracle(case):
+    """project (fastode/solvers.py:100-178) on the device against the oracle's
+    restatement, on a DenseSystem whose algebraic row is atan(y1 - y0) (a full
+    Newton step from y1 - y0 = 3 overshoots, so the line search halves it),
+    exp(y1 - y0) (no root: the projection fails and restores Y), or
+    an already consistent state (no Newton pass).  Status, every counter and
+    the state agree; four cells take different starting offsets."""
+    import fastode_amd as fa
+    import fastode_oracle as O
+    from fastode_amd.solvers import project
+
+    mass = np.array([1.0, 0.0])
+
+    def g_of(x):
+        return np.exp(x) if case == "no_root" else np.arctan(x)
+
+    def dg_of(x):
+        return np.exp(x) if case == "no_root" else 1.0 / (1.0 + x * x)
+
+    def rhs_np(Y):
+        F = np.empty_like(Y)
+        F[:, 0] = -Y[:, 0]
+        F[:, 1] = g_of(Y[:, 1] - Y[:, 0])
+        return F
+
+    def jac_np(Y):
+        x = Y[:, 1] - Y[:, 0]
+        J = np.zeros((Y.shape[0], 2, 2))
+        J[:, 0, 0] = -1.0
+        J[:, 1, 0] = -dg_of(x)
+        J[:, 1, 1] = dg_of(x)
+        return J
+
+    off = {"backtracks": [3.0, 2.5, -3.5, 1.2], "no_root": [0.5, 1.0, -2.0, 3.0],
+           "already_consistent": [0.0, 0.0, 0.0, 0.0]}[case]
+    Y0 = np.stack([np.full(4, 0.7), 0.7 + np.array(off)], axis=1)
+    atol, rtol = np.full(2, 1e-12), 1e-6
+    osys = O.DenseSystem(2, rhs_np, jac_np, mass=mass)
+    dsys = fa.DenseSystem(2, lambda Y: dev(rhs_np(Y.cpu().numpy())), lambda Y: dev(jac_np(Y.cpu().numpy())),
+                          mass=mass)
+    oc, c = O.Counts(), fa.Counts()
+    Yo, sto = O.project(osys, Y0.copy(), atol, rtol, oc)
+    Yd, st = project(dsys, dev(Y0), atol, rtol, c)
+    print(case, st, c.as_dict())
+    assert st == sto
+    assert c.as_dict() == oc.as_dict()
+    if case == "backtracks":
+        assert st == "converged" and c.proj_trials > c.proj_passes   # the line search halved a step
+        assert np.allclose(Yd.cpu().numpy(), Yo, rtol=1e-12, atol=1e-15)
+    elif case == "no_root":
+        assert st == "projection failed"
+        assert np.array_equal(Yd.cpu().numpy(), Y0)                  # restored
+    else:
+        assert st == "converged" and c.proj_passes == 1 and c.jac == 0
+        assert np.array_equal(Yd.cpu().numpy(), Y0)
