@@ -618,6 +618,28 @@ int fode_device_info(int device, int* lds_per_block, int* cus, int* warp_size) {
 int fode_mech_create(const fode_mech_desc* d, void** out) {
   if (!d || !out) return fail("fode_mech_create: null argument");
   if (d->n <= 0 || d->R < 0 || d->P <= 0 || d->nnz < d->n) return fail("fode_mech_create: bad sizes");
+  if (!d->react || !d->rhs_ptr || !d->jac_ptr || !d->jac_const || (d->ncon > 0 && (!d->con_rows || !d->con_coef ||
+                                                                                  !d->con_total)))
+    return fail("fode_mech_create: null table");
+  // Every index the kernels follow stays inside its array: reactant slots in
+  // [0, n] (n pads), forcing terms' reactions in [0, R), Jacobian terms'
+  // rate partials in [0, R*P), monotone row pointers.
+  for (long jp = 0; jp < (long)d->R * d->P; ++jp)
+    if (d->react[jp] < 0 || d->react[jp] > d->n) return fail("fode_mech_create: bad react entry");
+  if (d->rhs_ptr[0] != 0 || d->jac_ptr[0] != 0) return fail("fode_mech_create: row pointers must start at 0");
+  for (int i = 0; i < d->n; ++i)
+    if (d->rhs_ptr[i + 1] < d->rhs_ptr[i]) return fail("fode_mech_create: rhs_ptr not monotone");
+  for (int e = 0; e < d->nnz; ++e)
+    if (d->jac_ptr[e + 1] < d->jac_ptr[e]) return fail("fode_mech_create: jac_ptr not monotone");
+  if ((d->rhs_ptr[d->n] > 0 && (!d->rhs_rxn || !d->rhs_coef)) || (d->jac_ptr[d->nnz] > 0 && (!d->jac_src ||
+                                                                                            !d->jac_coef)))
+    return fail("fode_mech_create: null table");
+  for (int q = 0; q < d->rhs_ptr[d->n]; ++q)
+    if (d->rhs_rxn[q] < 0 || d->rhs_rxn[q] >= d->R) return fail("fode_mech_create: bad rhs_rxn");
+  for (int q = 0; q < d->jac_ptr[d->nnz]; ++q)
+    if (d->jac_src[q] < 0 || (long)d->jac_src[q] >= (long)d->R * d->P) return fail("fode_mech_create: bad jac_src");
+  for (int c = 0; c < d->ncon; ++c)
+    if (d->con_rows[c] < 0 || d->con_rows[c] >= d->n) return fail("fode_mech_create: bad constraint row");
   auto* h = new MechHandle();
   MechDev& m = h->d;
   m.n = d->n; m.R = d->R; m.P = d->P; m.nnz = d->nnz; m.ncon = d->ncon;
@@ -631,10 +653,7 @@ int fode_mech_create(const fode_mech_desc* d, void** out) {
   rc |= upload_nonempty(d->jac_coef, (size_t)d->jac_ptr[d->nnz], (double**)&m.jac_coef, h->owned);
   rc |= upload(d->jac_const, (size_t)d->nnz, (double**)&m.jac_const, h->owned);
   std::vector<int> con_of(d->n, -1);
-  for (int c = 0; c < d->ncon; ++c) {
-    if (d->con_rows[c] < 0 || d->con_rows[c] >= d->n) { delete h; return fail("fode_mech_create: bad constraint row"); }
-    con_of[d->con_rows[c]] = c;
-  }
+  for (int c = 0; c < d->ncon; ++c) con_of[d->con_rows[c]] = c;
   rc |= upload(con_of.data(), con_of.size(), (int**)&m.con_of_row, h->owned);
   {
     // S^T by reaction from the species-major forcing tables, and the reactant
@@ -642,10 +661,7 @@ int fode_mech_create(const fode_mech_desc* d, void** out) {
     const int nz = d->rhs_ptr[d->n];
     std::vector<int> st_ptr(d->R + 1, 0), st_sp(nz);
     std::vector<double> st_coef(nz);
-    for (int q = 0; q < nz; ++q) {
-      if (d->rhs_rxn[q] < 0 || d->rhs_rxn[q] >= d->R) { delete h; return fail("fode_mech_create: bad rhs_rxn"); }
-      ++st_ptr[d->rhs_rxn[q] + 1];
-    }
+    for (int q = 0; q < nz; ++q) ++st_ptr[d->rhs_rxn[q] + 1];
     for (int j = 0; j < d->R; ++j) st_ptr[j + 1] += st_ptr[j];
     std::vector<int> fill(st_ptr.begin(), st_ptr.end() - 1);
     for (int i = 0; i < d->n; ++i)

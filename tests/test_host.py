@@ -191,6 +191,42 @@ def test_abi_library_exports_every_declared_symbol():
     assert declared == set(L.EXPORTS)
 
 
+@pytest.mark.parametrize("bad", ["react", "rhs_rxn", "jac_src", "rhs_ptr", "con_rows"])
+def test_mech_create_refuses_out_of_range_tables(bad):
+    """fode_mech_create checks every index the kernels follow before it
+    touches the device, so a corrupt table is an error message, not an
+    out-of-bounds read on the GPU (no device is needed to see it)."""
+    import fastode_amd._lib as L
+    mech = synthetic(12, 20, seed=3, constraints=1).mech
+
+    def arr(a, dt):
+        return np.ascontiguousarray(np.asarray(a), dtype=dt)
+
+    t = dict(react=arr(mech.react, np.int32), rhs_ptr=arr(mech.rhs_ptr, np.int32),
+             rhs_rxn=arr(mech.rhs_rxn, np.int32), rhs_coef=arr(mech.rhs_coef, np.float64),
+             jac_ptr=arr(mech.jac_ptr, np.int32), jac_src=arr(mech.jac_src, np.int32),
+             jac_coef=arr(mech.jac_coef, np.float64), jac_const=arr(mech.jac_const, np.float64),
+             con_rows=arr(mech.constraint_rows, np.int32), con_coef=arr(mech.constraint_coef, np.float64),
+             con_total=arr(mech.constraint_total, np.float64))
+    if bad == "react":
+        t["react"][3, 0] = mech.n + 1
+    elif bad == "rhs_rxn":
+        t["rhs_rxn"][-1] = mech.R
+    elif bad == "jac_src":
+        t["jac_src"][0] = mech.R * mech.P
+    elif bad == "rhs_ptr":
+        t["rhs_ptr"][2], t["rhs_ptr"][3] = t["rhs_ptr"][3] + 1, t["rhs_ptr"][3]
+    else:
+        t["con_rows"][0] = -1
+    d = L.MechDesc(n=mech.n, R=mech.R, P=mech.P, nnz=mech.nnz, ncon=len(mech.constraint_rows),
+                   **{k: v.ctypes.data_as(ct.c_void_p) for k, v in t.items()})
+    h = ct.c_void_p()
+    assert L.lib.fode_mech_create(ct.byref(d), ct.byref(h)) == -1
+    assert h.value is None
+    msg = L.lib.fode_last_error().decode()
+    assert msg.startswith("fode_mech_create: ") and ("bad" in msg or "monotone" in msg), msg
+
+
 def test_ctrl_record_layout_matches_header():
     import fastode_amd._lib as L
     # 12 doubles, 10 int32, 13 int64 (include/fastode_hip.h: fode_group_ctrl)
