@@ -51,7 +51,7 @@ AdjArgs make_args(const PlanDev& P, const PlanDev& PT, const MechDev& M, int S) 
   A.P = P;
   A.PT = PT;
   A.M = M;
-  A.lay = make_layout(P, M, S, false);
+  A.lay = make_layout(P, M, S, STAGE_NONE);
   A.adj = A.lay.total;
   int end = A.adj + 2 * M.n + 2 * S * M.n + M.R;
   const int scr_at = A.lay.engine + engine_scr_offset(P);

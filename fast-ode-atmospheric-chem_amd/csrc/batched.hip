@@ -100,7 +100,7 @@ int launch(const BatchArgs& A, int blocks_per_cu, hipStream_t s) {
   hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, s, A.b.queue);
   if (blocks_per_cu >= 2 && A.P.tile <= OCC2_MAX_TILE) {
     BatchArgs B = A;
-    B.lay = make_layout(A.P, A.M, A.p.stages, false);
+    B.lay = make_layout(A.P, A.M, A.p.stages, STAGE_NONE);
     return launch_batched_occ2(B, blocks_per_cu, s);
   }
   return A.lay.tables > 0 ? launch_batched_tiles<true>(A, blocks_per_cu, s)

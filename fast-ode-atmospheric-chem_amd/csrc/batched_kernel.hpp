@@ -52,10 +52,10 @@ inline Layout choose_layout(const PlanDev& P, const MechDev& M, int S, int optio
   int dev = 0, maxlds = 0;
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
-  const Layout staged = make_layout(P, M, S, true);
+  const Layout staged = make_layout(P, M, S, STAGE_ALL);
   if (!(options & 1) && (size_t)staged.total * sizeof(double) + sizeof(fode_group_ctrl) <= (size_t)maxlds)
     return staged;
-  return make_layout(P, M, S, false);
+  return make_layout(P, M, S, STAGE_NONE);
 }
 
 // One attempt on one cell. Returns the cell's error partial sum (thread 0).
@@ -272,22 +272,24 @@ __device__ __forceinline__ bool project_group(const BatchArgs& A, const Tables& 
 // DAE: the tableau is constrained (projection at the start of each call);
 // a separate instantiation keeps the projection's registers out of the
 // unconstrained kernel.
-// STAGED: the tables are in LDS (copied at the start); a separate
-// instantiation, so that the compiler addresses them as LDS rather than
-// through generic pointers.
+// STAGE: which tables are in LDS (copied at the start): none, all, or the
+// solves' only; separate instantiations, so that the compiler addresses
+// them as LDS rather than through generic pointers.
 // WPE: waves per SIMD the registers are held to (1: as many registers as the
 // code wants, one wave per SIMD; 2: at most 256 per lane, so that two
 // workgroups share a CU when their LDS fits).
-template <int TILE, bool DAE, bool STAGED, int WPE>
+template <int STAGE>
+__device__ __forceinline__ Tables kernel_tables(const PlanDev& P, double* smem) {
+  if constexpr (STAGE == STAGE_ALL) return stage_tables(P, reinterpret_cast<uint32_t*>(smem));
+  if constexpr (STAGE == STAGE_SOLVE) return stage_solve_tables(P, reinterpret_cast<uint32_t*>(smem));
+  return P.t;
+}
+
+template <int TILE, bool DAE, int STAGE, int WPE>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void k_batched(BatchArgs A) {
   extern __shared__ double smem[];
   const int n = A.M.n, S = A.p.stages;
-  Tables T;
-  if constexpr (STAGED) {
-    T = stage_tables(A.P, reinterpret_cast<uint32_t*>(smem));
-  } else {
-    T = A.P.t;
-  }
+  const Tables T = kernel_tables<STAGE>(A.P, smem);
   const EngineLds L = carve(smem + A.lay.engine, A.P);
   const CellLds V = carve_cell(smem, A.lay, n, A.M.R, S);
   __shared__ fode_group_ctrl gs;
@@ -444,16 +446,11 @@ __device__ __forceinline__ void be_cell_pass(const BatchArgs& A, const Tables& T
 // on the persistent workgroups: per group, the step's start Yn in Y and the
 // Newton iterate Yn1 in work; each pass runs cell by cell and the group's
 // decision (be_after) follows the block reductions of the flags.
-template <int TILE, bool STAGED, int WPE>
+template <int TILE, int STAGE, int WPE>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void k_batched_be(BatchArgs A) {
   extern __shared__ double smem[];
   const int n = A.M.n, S = A.p.stages;
-  Tables T;
-  if constexpr (STAGED) {
-    T = stage_tables(A.P, reinterpret_cast<uint32_t*>(smem));
-  } else {
-    T = A.P.t;
-  }
+  const Tables T = kernel_tables<STAGE>(A.P, smem);
   const EngineLds L = carve(smem + A.lay.engine, A.P);
   const CellLds V = carve_cell(smem, A.lay, n, A.M.R, S);
   __shared__ fode_group_ctrl gs;
@@ -515,10 +512,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
   }
 }
 
-template <int TILE, bool DAE, bool STAGED, int WPE>
+template <int TILE, bool DAE, int STAGE, int WPE>
 int launch_batched_as(const BatchArgs& A, int blocks_per_cu, hipStream_t s) {
   const bool be = A.p.loop == FODE_LOOP_BACKWARD_EULER;
-  const void* kern = be ? (const void*)k_batched_be<TILE, STAGED, WPE> : (const void*)k_batched<TILE, DAE, STAGED, WPE>;
+  const void* kern = be ? (const void*)k_batched_be<TILE, STAGE, WPE> : (const void*)k_batched<TILE, DAE, STAGE, WPE>;
   const size_t bytes = (size_t)A.lay.total * sizeof(double);
   int dev = 0, cus = 0, maxlds = 0;
   (void)hipGetDevice(&dev);
@@ -539,9 +536,9 @@ int launch_batched_as(const BatchArgs& A, int blocks_per_cu, hipStream_t s) {
   if (A.max_blocks > 0 && blocks > A.max_blocks) blocks = A.max_blocks;
   if (blocks < 1) blocks = 1;
   if (be)
-    hipLaunchKernelGGL((k_batched_be<TILE, STAGED, WPE>), dim3(blocks), dim3(NT), bytes, s, A);
+    hipLaunchKernelGGL((k_batched_be<TILE, STAGE, WPE>), dim3(blocks), dim3(NT), bytes, s, A);
   else
-    hipLaunchKernelGGL((k_batched<TILE, DAE, STAGED, WPE>), dim3(blocks), dim3(NT), bytes, s, A);
+    hipLaunchKernelGGL((k_batched<TILE, DAE, STAGE, WPE>), dim3(blocks), dim3(NT), bytes, s, A);
   e = hipGetLastError();
   if (e != hipSuccess) return bfail(std::string("k_batched: ") + hipGetErrorString(e));
   return 0;
@@ -555,8 +552,8 @@ int launch_batched_tiles(const BatchArgs& A, int blocks_per_cu, hipStream_t s) {
   switch (A.P.tile) {
 #define X(T)                                                                  \
   case T:                                                                     \
-    return dae ? launch_batched_as<T, true, STAGED, 1>(A, blocks_per_cu, s)   \
-               : launch_batched_as<T, false, STAGED, 1>(A, blocks_per_cu, s);
+    return dae ? launch_batched_as<T, true, STAGED ? STAGE_ALL : STAGE_NONE, 1>(A, blocks_per_cu, s)   \
+               : launch_batched_as<T, false, STAGED ? STAGE_ALL : STAGE_NONE, 1>(A, blocks_per_cu, s);
     X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10)
 #undef X
   }

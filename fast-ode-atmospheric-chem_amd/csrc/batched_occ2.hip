@@ -1,18 +1,47 @@
 // The batched persistent kernel held to two waves per SIMD (at most 256
 // registers per lane), for register tiles up to OCC2_MAX_TILE, with the
-// plan's tables in memory: two workgroups, two cells, share a CU when their
-// LDS fits, and each hides the other's latency.
+// factorization's tables in memory: two workgroups, two cells, share a CU
+// when their LDS fits, and each hides the other's latency.  The solves'
+// tables go to LDS when the CU still holds blocks_per_cu workgroups with
+// them (a solve's level-scheduled stages then read their task lists from
+// LDS instead of memory); the arithmetic, and so the bits, are the same.
 #include "batched_kernel.hpp"
 
 namespace fode {
 
+namespace {
+
+// Workgroups of kernel kern that fit one CU with bytes of dynamic LDS.
+int resident_per_cu(const void* kern, size_t bytes) {
+  if (hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess) return 0;
+  int occ = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, NT, bytes) != hipSuccess) return 0;
+  return occ;
+}
+
+template <int TILE, bool DAE>
+int launch_occ2_tile(const BatchArgs& A, int blocks_per_cu, hipStream_t s) {
+  const bool be = A.p.loop == FODE_LOOP_BACKWARD_EULER;
+  BatchArgs B = A;
+  B.lay = make_layout(A.P, A.M, A.p.stages, STAGE_SOLVE);
+  const void* kern = be ? (const void*)k_batched_be<TILE, STAGE_SOLVE, 2>
+                        : (const void*)k_batched<TILE, DAE, STAGE_SOLVE, 2>;
+  const int want = blocks_per_cu < 2 ? 2 : blocks_per_cu;
+  if (resident_per_cu(kern, (size_t)B.lay.total * sizeof(double)) >= want)
+    return launch_batched_as<TILE, DAE, STAGE_SOLVE, 2>(B, blocks_per_cu, s);
+  B.lay = make_layout(A.P, A.M, A.p.stages, STAGE_NONE);
+  return launch_batched_as<TILE, DAE, STAGE_NONE, 2>(B, blocks_per_cu, s);
+}
+
+}  // namespace
+
 int launch_batched_occ2(const BatchArgs& A, int blocks_per_cu, hipStream_t s) {
   const bool dae = A.p.constrained != 0 && A.p.loop != FODE_LOOP_BACKWARD_EULER;
   switch (A.P.tile) {
-#define X(T)                                                               \
-  case T:                                                                  \
-    return dae ? launch_batched_as<T, true, false, 2>(A, blocks_per_cu, s) \
-               : launch_batched_as<T, false, false, 2>(A, blocks_per_cu, s);
+#define X(T)                                                         \
+  case T:                                                            \
+    return dae ? launch_occ2_tile<T, true>(A, blocks_per_cu, s)      \
+               : launch_occ2_tile<T, false>(A, blocks_per_cu, s);
     X(1) X(2) X(3) X(4) X(5)
 #undef X
   }

@@ -23,13 +23,18 @@ __host__ __device__ inline int cell_region_doubles(int n, int R, int S) {
   return 4 * n + S * n + R + 4 + 2;
 }
 
-// stage: copy the task tables into LDS (when they fit beside the cell's data).
-inline Layout make_layout(const PlanDev& P, const MechDev& M, int S, bool stage) {
+// Which of the plan's tables a kernel copies into LDS.
+enum { STAGE_NONE = 0, STAGE_ALL = 1, STAGE_SOLVE = 2 };
+
+// stage: STAGE_ALL copies every task table into LDS (when they fit beside
+// the cell's data), STAGE_SOLVE the solves' tables only.
+inline Layout make_layout(const PlanDev& P, const MechDev& M, int S, int stage) {
   Layout l;
-  const int words = table_words(P.n_f_tasks, P.f_nst, P.n_f_pairs, P.n_sf_tasks, P.sf_nst,
-                                P.n_sb_tasks, P.sb_nst, P.n_s_pairs,
-                                P.n_sch_pairs);
-  l.tables = stage ? (words + 1) / 2 : 0;
+  const int words = stage == STAGE_ALL
+                        ? table_words(P.n_f_tasks, P.f_nst, P.n_f_pairs, P.n_sf_tasks, P.sf_nst, P.n_sb_tasks,
+                                      P.sb_nst, P.n_s_pairs, P.n_sch_pairs)
+                        : solve_table_words(P.n_sf_tasks, P.sf_nst, P.n_sb_tasks, P.sb_nst, P.n_s_pairs);
+  l.tables = stage != STAGE_NONE ? (words + 1) / 2 : 0;
   l.engine = l.tables;
   int end = l.engine + engine_lds_doubles(P);
   const int scr = P.tile * NT;

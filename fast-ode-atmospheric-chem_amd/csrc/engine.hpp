@@ -225,6 +225,35 @@ __device__ inline Tables stage_tables(const PlanDev& P, uint32_t* w) {
   return t;
 }
 
+// Stage only the solves' tables (their task lists, stage bounds and pairs)
+// into LDS at w; the factorization's tables stay in memory.  For plans whose
+// solves run many small level-scheduled stages, each stage's chain of table
+// reads then starts from LDS, where the whole set fits beside two cells.
+__device__ inline Tables stage_solve_tables(const PlanDev& P, uint32_t* w) {
+  Tables t = P.t;
+  uint32_t* p = w;
+  auto take = [&](const void* src, int words) {
+    uint32_t* d = p;
+    if (words > 0) copy_words(d, static_cast<const uint32_t*>(src), words);
+    p += words;
+    return d;
+  };
+  t.sf_tasks = reinterpret_cast<const uint2*>(take(P.t.sf_tasks, 2 * P.n_sf_tasks));
+  t.sf_ptr = reinterpret_cast<const int*>(take(P.t.sf_ptr, P.sf_nst + 1));
+  if ((p - w) & 1) ++p;
+  t.sb_tasks = reinterpret_cast<const uint2*>(take(P.t.sb_tasks, 2 * P.n_sb_tasks));
+  t.sb_ptr = reinterpret_cast<const int*>(take(P.t.sb_ptr, P.sb_nst + 1));
+  t.s_pairs = take(P.t.s_pairs, P.n_s_pairs);
+  __syncthreads();
+  return t;
+}
+
+__host__ __device__ inline int solve_table_words(int n_sf_tasks, int sf_nst, int n_sb_tasks, int sb_nst, int n_s_pairs) {
+  int w = 2 * n_sf_tasks + sf_nst + 1;
+  w += w & 1;
+  return w + 2 * n_sb_tasks + sb_nst + 1 + n_s_pairs;
+}
+
 __host__ __device__ inline int table_words(int n_f_tasks, int f_nst, int n_f_pairs, int n_sf_tasks,
                                            int sf_nst, int n_sb_tasks, int sb_nst, int n_s_pairs,
                                            int n_sch_pairs) {

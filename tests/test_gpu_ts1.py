@@ -194,6 +194,31 @@ def test_batched_ts1_two_cells_per_cu(config, rtol):
         assert np.array_equal(a[0][step], b[0][step]), (config, step)
 
 
+@pytest.mark.parametrize("config,rtol", [("alg", 1e-3), ("be", 1e-6)])
+def test_batched_ts1_two_cells_per_cu_with_solve_tables_in_lds(config, rtol):
+    """The tuned TS1 plan (rho 0.7: 177 sparse pivots, TILE 2) leaves room on
+    a CU for two cells with the solves' tables copied into LDS, so the
+    two-waves-per-SIMD kernel reads its solve stages from LDS.  It keeps the
+    recorded Counts and the one-wave kernel's states bit for bit."""
+    from fastode_amd.device import DevicePlan
+    mech = ts1().mech
+    saved = mech.__dict__.get("_plan")
+    mech._plan = DevicePlan(mech.n, mech.rows, mech.cols, rho=0.7, m_max=160)
+    try:
+        assert (mech._plan.plan.c, mech._plan.plan.tile) == (177, 2)
+        a = run(config, rtol, "tier1-30s", batched=True, blocks_per_cu=2)
+        b = run(config, rtol, "tier1-30s", batched=True, blocks_per_cu=1)
+    finally:
+        if saved is None:
+            del mech._plan
+        else:
+            mech._plan = saved
+    ref = runs_row(config, rtol, "tier1-30s")
+    assert {c: getattr(a[1], c) for c in COUNT_COLS} == {c: int(ref[c]) for c in COUNT_COLS}
+    for step in a[0]:
+        assert np.array_equal(a[0][step], b[0][step]), (config, step)
+
+
 def family_row(rtol, form, tableau, solver):
     with open(GOLD / "ts1_families.csv", newline="") as f:
         for r in csv.DictReader(f):
