@@ -1,10 +1,10 @@
 // The batched persistent kernel held to two waves per SIMD (at most 256
 // registers per lane), for register tiles up to OCC2_MAX_TILE, with the
-// factorization's tables in memory: two workgroups, two cells, share a CU
-// when their LDS fits, and each hides the other's latency.  The solves'
-// tables go to LDS when the CU still holds blocks_per_cu workgroups with
-// them (a solve's level-scheduled stages then read their task lists from
-// LDS instead of memory); the arithmetic, and so the bits, are the same.
+// plan's tables where they fit: two workgroups, two cells, share a CU and
+// each hides the other's latency.  All tables go to LDS when the CU still
+// holds blocks_per_cu workgroups with them, else the solves' tables only (a
+// solve's level-scheduled stages then read their task lists from LDS instead
+// of memory), else none; the arithmetic, and so the bits, are the same.
 #include "batched_kernel.hpp"
 
 namespace fode {
@@ -19,15 +19,25 @@ int resident_per_cu(const void* kern, size_t bytes) {
   return occ;
 }
 
+// Whether blocks_per_cu workgroups of the STAGE kernel fit one CU; B.lay
+// becomes that kernel's layout.
+template <int TILE, bool DAE, int STAGE>
+bool occ2_fits(BatchArgs& B, int blocks_per_cu) {
+  const bool be = B.p.loop == FODE_LOOP_BACKWARD_EULER;
+  B.lay = make_layout(B.P, B.M, B.p.stages, STAGE);
+  const void* kern = be ? (const void*)k_batched_be<TILE, STAGE, 2> : (const void*)k_batched<TILE, DAE, STAGE, 2>;
+  const int want = blocks_per_cu < 2 ? 2 : blocks_per_cu;
+  return resident_per_cu(kern, (size_t)B.lay.total * sizeof(double)) >= want;
+}
+
+// Every table in LDS when that still fits blocks_per_cu workgroups on a CU,
+// else the solves' tables, else none.
 template <int TILE, bool DAE>
 int launch_occ2_tile(const BatchArgs& A, int blocks_per_cu, hipStream_t s) {
-  const bool be = A.p.loop == FODE_LOOP_BACKWARD_EULER;
   BatchArgs B = A;
-  B.lay = make_layout(A.P, A.M, A.p.stages, STAGE_SOLVE);
-  const void* kern = be ? (const void*)k_batched_be<TILE, STAGE_SOLVE, 2>
-                        : (const void*)k_batched<TILE, DAE, STAGE_SOLVE, 2>;
-  const int want = blocks_per_cu < 2 ? 2 : blocks_per_cu;
-  if (resident_per_cu(kern, (size_t)B.lay.total * sizeof(double)) >= want)
+  if (occ2_fits<TILE, DAE, STAGE_ALL>(B, blocks_per_cu))
+    return launch_batched_as<TILE, DAE, STAGE_ALL, 2>(B, blocks_per_cu, s);
+  if (occ2_fits<TILE, DAE, STAGE_SOLVE>(B, blocks_per_cu))
     return launch_batched_as<TILE, DAE, STAGE_SOLVE, 2>(B, blocks_per_cu, s);
   B.lay = make_layout(A.P, A.M, A.p.stages, STAGE_NONE);
   return launch_batched_as<TILE, DAE, STAGE_NONE, 2>(B, blocks_per_cu, s);

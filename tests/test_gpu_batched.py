@@ -129,6 +129,54 @@ def test_batched_constrained_equals_plain_bitwise(stage_tables):
     assert rec["counts"][:, 10].min() >= 2          # the projection did a Newton pass
 
 
+@pytest.mark.parametrize("rho,cut", [(0.75, 22), (0.85, 29)], ids=["all-tables-in-lds", "solve-tables-in-lds"])
+def test_two_cells_per_cu_constrained_equals_plain_bitwise(rho, cut):
+    """Two cells per CU on the constrained workload.  Under the tuned plan
+    (22 sparse pivots, TILE 5) every table fits in LDS beside two cells;
+    under the denser split (29 pivots) only the solves' tables do.  Either
+    way the states, Counts and step proposals are the plain path's, bit for
+    bit, on a ragged batch whose projection takes a Newton update."""
+    import fastode_amd as fa
+    from fastode_amd import workloads as W
+    from fastode_amd.device import DevicePlan
+    from fastode_amd.system import ChemSystem
+    wl = W.constrained100(C=40)
+    mech = wl.mech
+    saved = mech.__dict__.get("_plan")
+    mech._plan = DevicePlan(mech.n, mech.rows, mech.cols, rho=rho, m_max=160)
+    try:
+        assert (mech._plan.plan.c, mech._plan.plan.tile) == (cut, 5)
+        Y0 = wl.Y0.copy()
+        Y0[:, mech.constraint_rows] *= 1.0 + 1e-3 * np.arange(1, 41)[:, None]
+        k = static_k(wl, np.arange(40))
+        system = ChemSystem(mech, k, mass=wl.mass)
+        run = fa.BatchedAlgorithm(fa.RODAS3, blocks_per_cu=2).runner(system, Y0, wl.dt, wl.atol, wl.rtol,
+                                                                      graph=False)
+        alg = fa.Algorithm(fa.RODAS3)
+        mems = [alg.new_memory() for _ in range(2)]
+        cnt = [fa.Counts() for _ in range(2)]
+        Yp = Y0.copy()
+        for it in range(2):
+            run.step(None)
+            for gi, sl in enumerate([slice(0, 32), slice(32, 40)]):
+                Yp[sl], st = alg.solve(ChemSystem(mech, k[sl], mass=wl.mass), Yp[sl], wl.dt, wl.atol,
+                                       wl.rtol, cnt[gi], mems[gi])
+                assert st == "converged"
+            assert np.array_equal(run.Y.cpu().numpy(), Yp)
+        rec = run.memory.read()
+    finally:
+        if saved is None:
+            del mech._plan
+        else:
+            mech._plan = saved
+    for gi in range(2):
+        assert list(rec["counts"][gi]) == [getattr(cnt[gi], f) for f in
+                                           ("steps", "accepted", "rejected", "rhs", "jac", "lu", "solves",
+                                            "newton", "failures", "unconverged", "proj_passes",
+                                            "proj_trials", "calls")]
+        assert rec["h_next"][gi] == mems[gi].h_next
+
+
 def test_batched_equals_plain_bitwise_and_graph_replays_match_eager():
     """Groups of uneven size (the last one short), 3 intervals: the batched
     states equal the plain path's bit for bit, with eager work between graph
