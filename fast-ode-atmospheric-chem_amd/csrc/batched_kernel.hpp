@@ -412,7 +412,7 @@ __device__ __forceinline__ void be_cell_pass(const BatchArgs& A, const Tables& T
     for (int j = tid; j < M.R; j += NT) V.kc[j] = kg[j] * B.kmul[j];
   __syncthreads();
   rates_into<slot3_paths(TILE)>(M, V.y, V.kc, V.rd);
-  forcing_into(M, V.y, V.rd, V.f0);
+  forcing_into<slot3_paths(TILE)>(M, V.y, V.rd, V.f0);
   jacobian_into<slot3_paths(TILE)>(M, V);
   double a[TILE][TILE];
   assemble<TILE>(P, JLds{V.jv}, g.alpha, B.mass, false, L.lds, a);

@@ -142,13 +142,14 @@ __device__ inline void rates_into(const MechDev& M, const double* y, const doubl
   __syncthreads();
 }
 
+template <bool WIDE = true>
 __device__ inline void forcing_into(const MechDev& M, const double* y, const double* rates, double* out) {
   for (int i = threadIdx.x; i < M.n; i += NT) {
     if (M.con_of_row[i] >= 0) {
       out[i] = forcing(M, y, rates, i);
       continue;
     }
-    out[i] = ordered_dot(0.0, rates, M.rhs_rxn, M.rhs_coef, M.rhs_ptr[i], M.rhs_ptr[i + 1]);
+    out[i] = ordered_dot<WIDE>(0.0, rates, M.rhs_rxn, M.rhs_coef, M.rhs_ptr[i], M.rhs_ptr[i + 1]);
   }
   __syncthreads();
 }
@@ -213,7 +214,7 @@ __device__ inline void jacobian_into(const MechDev& M, const CellLds& V) {
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      if (qa[u] + 1 < qb[u]) v[u] = ordered_dot(v[u], V.rd, M.jac_src, M.jac_coef, qa[u] + 1, qb[u]);
+      if (qa[u] + 1 < qb[u]) v[u] = ordered_dot<SLOT3>(v[u], V.rd, M.jac_src, M.jac_coef, qa[u] + 1, qb[u]);
       if (e0 + u * NT < M.nnz) V.jv[e0 + u * NT] = v[u] + c[u];
     }
   }
@@ -231,7 +232,7 @@ __device__ __forceinline__ void cell_stages(const PlanDev& P, const Tables& T, c
   // F(Y) (fastode/solvers.py:345) and the Jacobian (:347), recomputed per
   // attempt: they are the same values a retry reuses.
   rates_into<slot3_paths(TILE)>(M, V.y, V.kc, V.rd);
-  forcing_into(M, V.y, V.rd, V.f0);
+  forcing_into<slot3_paths(TILE)>(M, V.y, V.rd, V.f0);
   FODE_PHASE(PH_RATES);
   jacobian_into<slot3_paths(TILE)>(M, V);
   FODE_PHASE(PH_JACOBIAN);
@@ -256,7 +257,7 @@ __device__ __forceinline__ void cell_stages(const PlanDev& P, const Tables& T, c
       }
       __syncthreads();
       rates_into<slot3_paths(TILE)>(M, V.w, V.kc, V.rd);
-      forcing_into(M, V.w, V.rd, V.fr);
+      forcing_into<slot3_paths(TILE)>(M, V.w, V.rd, V.fr);
       forcing_src = V.fr;
     }
     // right-hand side of stage r, gathered into the solve vector (new order)

@@ -35,9 +35,30 @@ __device__ inline double drate(const MechDev& M, const double* y, const double* 
 // terms in flight together: a long row (TS1 has species with 192 terms and a
 // Jacobian entry with 110) costs one load latency per TERMS terms, not per
 // term.
+// WIDE (kernels with registers to spare): rounds of 2*TERMS terms first, so
+// that a long row (TS1's longest forcing row has 192 terms) waits on half as
+// many load latencies; the additions keep the term order, so the bits are
+// those of the narrow rounds.
 constexpr int TERMS = 8;
+template <bool WIDE = false>
 __device__ inline double ordered_dot(double f, const double* vals, const int* src, const double* coef, int q,
                                      int qe) {
+  if constexpr (WIDE) {
+    constexpr int W = 2 * TERMS;
+    for (; q + W <= qe; q += W) {
+      int x[W];
+      double c[W], r[W];
+#pragma unroll
+      for (int u = 0; u < W; ++u) {
+        x[u] = src[q + u];
+        c[u] = coef[q + u];
+      }
+#pragma unroll
+      for (int u = 0; u < W; ++u) r[u] = vals[x[u]];
+#pragma unroll
+      for (int u = 0; u < W; ++u) f = f + r[u] * c[u];
+    }
+  }
   for (; q + TERMS <= qe; q += TERMS) {
     int x[TERMS];
     double c[TERMS], r[TERMS];
