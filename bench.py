@@ -393,6 +393,7 @@ def main():
                 "achieved_is": "algorithmic bytes (the compulsory HBM traffic of a streaming, unfused "
                                "implementation of the same Counts: LU written and read per attempt) per "
                                "launch / kernel time: a work-rate index, not bytes this kernel moves",
+                "alg_work_rate_gbs": round(achieved, 1),
                 "alg_bytes_per_launch": round(alg_bytes / K), "avg_launch_ms": round(1e3 * kernel_s / K, 3),
                 "hbm_counter_gbs": None if counter_gbs is None else round(counter_gbs, 2),
                 "hbm_counter_frac": None if counter_gbs is None else round(counter_gbs / HBM_PEAK_GBS, 5),
