@@ -132,14 +132,21 @@ def clamp(system, Y):
 def project(system, Y, atol, rtol, counts: Counts, max_updates=10, tol=0.1,
             max_backtracks=24, backtrack_factor=0.5, sigma=1.0e-4):
     """Consistent initialization by damped Newton on the algebraic rows
-    (fastode/solvers.py:100-178), on device tensors.  Returns (Y, status);
-    a failure restores Y.
+    (fastode/solvers.py:100-178).  Returns (Y, status); a failure restores
+    Y.  A NumPy Y comes back as NumPy, a device tensor as a device tensor.
 
     The residual masking, the backtracking candidates and the finite /
     zero / weighted-max measures run as HIP kernels (``fode_proj_*``); the
     host takes the reference's decisions, reading two doubles per measure."""
     if not np.any(system.mass_np == 0.0):
         return Y, "converged"
+    as_numpy = not isinstance(Y, torch.Tensor)
+    Yp, status = _project_device(system, as_device(Y), atol, rtol, counts, max_updates, tol, max_backtracks,
+                                 backtrack_factor, sigma)
+    return _out(Yp, as_numpy), status
+
+
+def _project_device(system, Y, atol, rtol, counts, max_updates, tol, max_backtracks, backtrack_factor, sigma):
     C, n = Y.shape
     s = L.stream_handle()
     mass = system.mass_dev

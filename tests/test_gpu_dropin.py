@@ -306,6 +306,8 @@ def test_project_newton_and_backtracking_match_the_oracle(case):
     print(case, st, c.as_dict())
     assert st == sto
     assert c.as_dict() == oc.as_dict()
+    Yn, stn = project(dsys, Y0.copy(), atol, rtol, fa.Counts())     # NumPy in, NumPy out
+    assert stn == st and isinstance(Yn, np.ndarray) and np.array_equal(Yn, Yd.cpu().numpy())
     if case == "backtracks":
         assert st == "converged" and c.proj_trials > c.proj_passes   # the line search halved a step
         assert np.allclose(Yd.cpu().numpy(), Yo, rtol=1e-12, atol=1e-15)
