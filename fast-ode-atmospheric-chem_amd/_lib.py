@@ -134,7 +134,7 @@ _SIGS = {
     "fode_stage_point": (i32, [P(CtrlParams), i32, i32, i32, vp, P(vp), vp, vp]),
     "fode_stage_forcing": (i32, [P(CtrlParams), i32, i32, i32, vp, vp, P(vp), vp, i32, vp, vp]),
     "fode_stage_final": (i32, [P(CtrlParams), i32, i32, vp, P(vp), vp, vp, vp]),
-    "fode_norm_control": (i32, [P(CtrlParams), vp, i32, i32, vp, vp, vp, vp, ct.c_double, vp, vp]),
+    "fode_norm_control": (i32, [P(CtrlParams), vp, i32, i32, vp, vp, vp, vp, ct.c_double, vp, vp, vp]),
     "fode_clamp": (i32, [i32, i32, vp, vp, i32, vp]),
     "fode_proj_mask": (i32, [i32, i32, vp, vp, vp, vp]),
     "fode_proj_candidate": (i32, [i32, i32, vp, vp, ct.c_double, vp, vp, vp]),

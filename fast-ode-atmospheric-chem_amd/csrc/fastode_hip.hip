@@ -374,6 +374,60 @@ __global__ __launch_bounds__(256) void k_norm_control(fode_ctrl_params p, fode_g
   }
 }
 
+// The same norm and controller over three launches, for groups whose cells
+// are many enough that one workgroup's loads in flight bound the kernel above.
+// k_norm_cells: workgroup c forms cell c's share exactly as k_norm_control
+// forms it (thread t's entries t, t + NT, ..., the wave's xor-shuffle tree,
+// the wave sums in order) into share[c].
+__global__ __launch_bounds__(256) void k_norm_cells(fode_ctrl_params p, int n, const double* __restrict__ Y,
+                                                    const double* __restrict__ Ynew, const double* __restrict__ E,
+                                                    const double* __restrict__ atol, double rtol,
+                                                    const double* __restrict__ mass, double* __restrict__ share) {
+  constexpr int WAVES = NT / 64;
+  __shared__ double part[WAVES];
+  const int c = blockIdx.x;
+  const bool step_rows = step_change_rows(p);
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += NT) {
+    const size_t o = (size_t)c * n + i;
+    const bool alg = step_rows && mass[i] == 0.0;
+    const double y = Y[o], yn = Ynew[o];
+    const double e = alg ? yn - y : E[o];
+    const double s = e / (atol[i] + rtol * fmax(fabs(y), fabs(yn)));
+    acc = acc + s * s;
+  }
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int w = 0; w < WAVES; ++w) s += part[w];
+    share[c] = s;
+  }
+}
+
+// The cells' shares added in cell order, then the controller (one thread).
+__global__ void k_norm_decide(fode_ctrl_params p, fode_group_ctrl* g, int C, int n, const double* __restrict__ share) {
+  if (threadIdx.x != 0) return;
+  double sum = 0.0;
+  for (int c = 0; c < C; ++c) sum = sum + share[c];
+  fode_group_ctrl gc = *g;
+  ctrl_after(p, gc, err_from_sum(sum, (double)C * n));
+  *g = gc;
+}
+
+// On acceptance Y <- Ynew; on finishing, the clamp of the result.
+__global__ void k_norm_apply(long N, int n, const fode_group_ctrl* __restrict__ g, double* __restrict__ Y,
+                             const double* __restrict__ Ynew, const double* __restrict__ mass, int constrained) {
+  const bool accepted = g->accepted != 0, finish = g->action == FODE_FINISH;
+  if (!accepted && !finish) return;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+    double v = accepted ? Ynew[i] : Y[i];
+    if (finish && !(constrained && mass[i % n] != 1.0)) v = np_max0(v);
+    Y[i] = v;
+  }
+}
+
 // Backward Euler's Newton right-hand side F - (Yn1 - Yn) / H
 // (fastode/solvers.py:456), H from the group's control block.
 __global__ void k_be_residual(long N, const fode_group_ctrl* __restrict__ g, const double* __restrict__ F,
@@ -873,10 +927,20 @@ int fode_stage_final(const fode_ctrl_params* p, int C, int n, const double* Y, c
 
 int fode_norm_control(const fode_ctrl_params* p, fode_group_ctrl* g, int C, int n, double* Y,
                       const double* Ynew, const double* E, const double* atol, double rtol,
-                      const double* mass, void* stream) {
+                      const double* mass, double* work, void* stream) {
   if (!p || !g || C <= 0 || n <= 0) return fail("fode_norm_control: bad argument");
-  hipLaunchKernelGGL(k_norm_control, dim3(1), dim3(256), 0, S(stream), *p, g, C, n, Y, Ynew, E, atol,
-                     rtol, mass);
+  if (!work) {
+    hipLaunchKernelGGL(k_norm_control, dim3(1), dim3(256), 0, S(stream), *p, g, C, n, Y, Ynew, E, atol,
+                       rtol, mass);
+    HIP_OK(hipGetLastError());
+    return 0;
+  }
+  const long N = (long)C * n;
+  hipLaunchKernelGGL(k_norm_cells, dim3(C), dim3(256), 0, S(stream), *p, n, Y, Ynew, E, atol, rtol, mass, work);
+  HIP_OK(hipGetLastError());
+  hipLaunchKernelGGL(k_norm_decide, dim3(1), dim3(64), 0, S(stream), *p, g, C, n, work);
+  HIP_OK(hipGetLastError());
+  hipLaunchKernelGGL(k_norm_apply, dim3(grid_for(N)), dim3(256), 0, S(stream), N, n, g, Y, Ynew, mass, p->constrained);
   HIP_OK(hipGetLastError());
   return 0;
 }

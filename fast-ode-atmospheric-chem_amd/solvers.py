@@ -289,6 +289,7 @@ def rosenbrock_loop(self, system, Y, dt, atol, rtol, counts: Counts, memory: Mem
         Yd = Yp.contiguous()
         L.check(L.lib.fode_ctrl_begin(ct.byref(params), ctrl.ptr, 1, s), "ctrl_begin")
     work = {k: torch.empty_like(Yd) for k in ("Yr", "rhs", "Ynew", "E")}
+    shares = torch.empty(C, dtype=F64, device=Yd.device)     # the cells' shares of the error norm
     K = [torch.empty_like(Yd) for _ in range(tab.stages)]
     g = ctrl.read()
     F0 = J = None
@@ -303,7 +304,7 @@ def rosenbrock_loop(self, system, Y, dt, atol, rtol, counts: Counts, memory: Mem
         Ynew, E = stages(system, tab, params, Yd, F0, LU, ctrl, K, work)
         L.check(L.lib.fode_norm_control(ct.byref(params), ctrl.ptr, C, n, L.ptr(Yd), L.ptr(Ynew),
                                         L.ptr(E), L.ptr(atol_d), float(rtol),
-                                        L.ptr(system.mass_dev), s), "fode_norm_control")
+                                        L.ptr(system.mass_dev), L.ptr(shares), s), "fode_norm_control")
         g = ctrl.read()
         if self.trace is not None:
             self.trace.append((H, float(g["err"]), bool(g["accepted"])))

@@ -177,10 +177,12 @@ int fode_stage_final(const fode_ctrl_params* p, int C, int n, const double* Y,
 /* error_norm (fastode/solvers.py:37-43) over the group, then the
  * accept/reject update of Algorithm.solve (fastode/solvers.py:362-411) on
  * the device; on acceptance Y <- Ynew, on finishing Y is clamped
- * (fastode/solvers.py:89-97, 412-415). */
+ * (fastode/solvers.py:89-97, 412-415).  work: C doubles of device memory
+ * (the cells' shares of the norm, one workgroup per cell), or NULL for one
+ * workgroup over the whole group; both give the same bits. */
 int fode_norm_control(const fode_ctrl_params* p, fode_group_ctrl* g, int C, int n, double* Y,
                       const double* Ynew, const double* E, const double* atol, double rtol,
-                      const double* mass, void* stream);
+                      const double* mass, double* work, void* stream);
 
 /* Backward Euler's Newton pass around the caller's rhs/jac/factor/solve
  * (BackwardEuler.solve, fastode/solvers.py:448-461): out = F - (Yn1 - Yn)/H

@@ -189,3 +189,51 @@ def test_stage_kernels_match_the_reference_formulas():
                 we += tab.e[r] * K[r]
         assert np.array_equal(Ynew.cpu().numpy(), wy) and np.array_equal(E.cpu().numpy(), we)
         _ = math
+
+
+@pytest.mark.parametrize("loop", ["algorithm", "restarting"])
+def test_norm_control_per_cell_shares_equal_one_workgroup(loop):
+    """fode_norm_control with a work array (one workgroup per cell, the
+    shares added in cell order, then the controller and the accept/clamp
+    over a grid) against the one-workgroup form: the same control block and
+    the same state, bit for bit, across accepting, rejecting and finishing
+    attempts.  A restarting loop with a constraint row exercises the
+    step-change rows of the error."""
+    import ctypes as ct
+    import fastode_amd._lib as L
+    from fastode_amd import Algorithm, Memory, RestartingLoop, ROS3
+    from fastode_amd.solvers import GroupControl, ctrl_params
+    rng = np.random.default_rng(11)
+    C, n = 40, 300                      # more species than threads, a ragged cell count
+    mass = np.ones(n)
+    constrained = loop == "restarting"
+    if constrained:
+        mass[[3, 77]] = 0.0
+    solver = RestartingLoop(ROS3, step_change_rows=True) if constrained else Algorithm(ROS3)
+    p = ctrl_params(solver, ROS3, constrained)
+    atol, rtol = dev(np.full(n, 1e-8)), 1e-3
+    md = dev(mass)
+    s = L.stream_handle()
+    seen = set()
+    for trial, scale in enumerate((1e-6, 1e3, 1e-6, 1e-2)):
+        Y = rng.uniform(-1e-3, 1.0, (C, n))      # some negative entries for the clamp
+        Ynew = Y + scale * rng.standard_normal((C, n))
+        E = scale * rng.standard_normal((C, n))
+        outs = []
+        for use_work in (False, True):
+            mem = Memory()
+            if trial == 2:
+                mem.h_next = 1.0        # the carried step covers the interval: the attempt finishes
+            ctrl = GroupControl(1.0, mem)
+            L.check(L.lib.fode_ctrl_init(ct.byref(p), ctrl.ptr, 1, 0, s))
+            Yd, Ynd, Ed = dev(Y), dev(Ynew), dev(E)
+            work = torch.empty(C, dtype=torch.float64, device="cuda") if use_work else None
+            L.check(L.lib.fode_norm_control(ct.byref(p), ctrl.ptr, C, n, L.ptr(Yd), L.ptr(Ynd), L.ptr(Ed),
+                                            L.ptr(atol), rtol, L.ptr(md), L.ptr(work), s))
+            outs.append((ctrl.dev.cpu().numpy().copy(), Yd.cpu().numpy()))
+            rec = ctrl.read()
+        (g0, y0), (g1, y1) = outs
+        assert np.array_equal(g0, g1), (loop, trial)
+        assert np.array_equal(y0, y1), (loop, trial)
+        seen.add("finish" if rec["action"] == L.FINISH else ("accept" if rec["accepted"] else "reject"))
+    assert seen >= ({"accept", "reject", "finish"} if loop == "algorithm" else {"accept", "reject"}), seen
