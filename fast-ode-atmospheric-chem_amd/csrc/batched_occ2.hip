@@ -11,11 +11,21 @@ namespace fode {
 
 namespace {
 
-// Workgroups of kernel kern that fit one CU with bytes of dynamic LDS.
+// Workgroups of kernel kern that fit one CU with bytes of dynamic LDS.  A
+// layout beyond the device's limit is 0 without asking the runtime, and an
+// error of the query is cleared here, so that the launch's own
+// hipGetLastError() does not report it.
 int resident_per_cu(const void* kern, size_t bytes) {
-  if (hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess) return 0;
+  int dev = 0, maxlds = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+  if (bytes + sizeof(fode_group_ctrl) > (size_t)maxlds) return 0;
   int occ = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, NT, bytes) != hipSuccess) return 0;
+  if (hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, NT, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
   return occ;
 }
 
