@@ -322,6 +322,7 @@ int blocks_for(const AdjArgs& A, int C, int* blocks, size_t* bytes) {
   if (e != hipSuccess) return afail(std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
   int occ = 0;
   e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_adjoint<TILE>, NT, *bytes);
+  if (e != hipSuccess) (void)hipGetLastError();   // not the launch's error
   if (e != hipSuccess || occ < 1) occ = 1;
   int b = cus * occ;
   if (b > C) b = C;

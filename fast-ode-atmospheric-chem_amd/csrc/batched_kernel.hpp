@@ -528,6 +528,7 @@ int launch_batched_as(const BatchArgs& A, int blocks_per_cu, hipStream_t s) {
   if (e != hipSuccess) return bfail(std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
   int occ = 0;
   e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, NT, bytes);
+  if (e != hipSuccess) (void)hipGetLastError();   // not the launch's error
   if (e != hipSuccess || occ < 1) occ = 1;
   // never more persistent workgroups than can be resident at once
   const int per_cu = blocks_per_cu > 0 && blocks_per_cu < occ ? blocks_per_cu : occ;
