@@ -174,6 +174,14 @@ struct JGlobal {
   __device__ double operator()(int e) const { return J[e]; }
 };
 
+// Doubles of LDS ahead of the engine's region when k_solve copies the
+// solves' tables there.
+__host__ __device__ inline int solve_table_doubles(const PlanDev& P) {
+  return (solve_table_words(P.n_sf_tasks, P.sf_nst, P.n_sb_tasks, P.sb_nst, P.n_s_pairs) + 1) / 2;
+}
+
+// The factorization reads its tables from memory: copying them (49 KB for
+// TS1's plan) into each cell's LDS cost more than it saved (52.1 -> 53.2 us).
 template <int TILE>
 __global__ __launch_bounds__(NT) void k_factor(PlanDev P, const double* __restrict__ alpha_p,
                                                const double* __restrict__ mass,
@@ -197,11 +205,20 @@ __global__ __launch_bounds__(NT) void k_factor(PlanDev P, const double* __restri
   for (int i = tid; i < 2 * TILE * NT; i += NT) dd[i] = L.dL[i];   // dL and dU are adjacent
 }
 
-template <int TILE>
+// STAGED: the solves' tables are copied into LDS ahead of the engine's
+// region (a separate instantiation, so that they are addressed as LDS); the
+// arithmetic, and so the bits, are the same.
+template <int TILE, bool STAGED>
 __global__ __launch_bounds__(NT) void k_solve(PlanDev P, const double* __restrict__ LU,
                                               const double* B, double* X) {
   extern __shared__ double smem[];
-  const EngineLds L = carve(smem, P);
+  Tables T = P.t;
+  int off = 0;
+  if constexpr (STAGED) {
+    T = stage_solve_tables(P, reinterpret_cast<uint32_t*>(smem));
+    off = solve_table_doubles(P);
+  }
+  const EngineLds L = carve(smem + off, P);
   const int cell = blockIdx.x, tid = threadIdx.x;
   const double* in = LU + (size_t)cell * P.lu_len;
   double a[TILE][TILE];
@@ -216,7 +233,7 @@ __global__ __launch_bounds__(NT) void k_solve(PlanDev P, const double* __restric
   double* vec = L.lds + P.vec;
   const double* b = B + (size_t)cell * P.n;
   for (int i = tid; i < P.c + P.mpad; i += NT) vec[i] = (i < P.n) ? b[P.perm[i]] : 0.0;
-  solve_cell<TILE>(P, P.t, L, a);
+  solve_cell<TILE>(P, T, L, a);
   double* x = X + (size_t)cell * P.n;
   for (int i = tid; i < P.n; i += NT) x[P.perm[i]] = vec[i];
 }
@@ -621,6 +638,14 @@ size_t engine_bytes(const PlanDev& P) {
   return (size_t)engine_lds_doubles(P) * sizeof(double);
 }
 
+// Whether bytes of dynamic LDS fit one workgroup of the device.
+bool fits_workgroup(size_t bytes) {
+  int dev = 0, maxlds = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+  return bytes <= (size_t)maxlds;
+}
+
 template <int TILE>
 int launch_factor(const PlanDev& P, int C, const double* alpha, const double* mass, const double* J,
                   int alg_only, double* LU, hipStream_t s) {
@@ -632,12 +657,21 @@ int launch_factor(const PlanDev& P, int C, const double* alpha, const double* ma
   return 0;
 }
 
+// k_solve (one workgroup per cell) copies the solves' tables into LDS
+// whenever they fit beside the engine.
 template <int TILE>
 int launch_solve(const PlanDev& P, int C, const double* LU, const double* B, double* X, hipStream_t s) {
-  const size_t bytes = engine_bytes(P);
-  HIP_OK(hipFuncSetAttribute((const void*)k_solve<TILE>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)bytes));
-  hipLaunchKernelGGL(k_solve<TILE>, dim3(C), dim3(NT), bytes, s, P, LU, B, X);
+  const size_t staged = engine_bytes(P) + (size_t)solve_table_doubles(P) * sizeof(double);
+  if (fits_workgroup(staged)) {
+    HIP_OK(hipFuncSetAttribute((const void*)k_solve<TILE, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)staged));
+    hipLaunchKernelGGL((k_solve<TILE, true>), dim3(C), dim3(NT), staged, s, P, LU, B, X);
+  } else {
+    const size_t bytes = engine_bytes(P);
+    HIP_OK(hipFuncSetAttribute((const void*)k_solve<TILE, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)bytes));
+    hipLaunchKernelGGL((k_solve<TILE, false>), dim3(C), dim3(NT), bytes, s, P, LU, B, X);
+  }
   HIP_OK(hipGetLastError());
   return 0;
 }
