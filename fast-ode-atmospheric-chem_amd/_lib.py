@@ -141,7 +141,7 @@ _SIGS = {
     "fode_proj_measure": (i32, [i32, i32, vp, vp, vp, vp, ct.c_double, vp, vp]),
     "fode_be_residual": (i32, [i32, i32, vp, vp, vp, vp, vp, vp]),
     "fode_scale_columns": (i32, [i32, i32, vp, i32, vp, vp, vp, vp, vp]),
-    "fode_be_update": (i32, [P(CtrlParams), vp, i32, i32, vp, vp, vp, vp, ct.c_double, vp]),
+    "fode_be_update": (i32, [P(CtrlParams), vp, i32, i32, vp, vp, vp, vp, ct.c_double, vp, vp]),
     "fode_batched_interval": (i32, [vp, vp, P(CtrlParams), P(BatchDesc), i32, vp]),
     "fode_batched_lds_bytes": (i32, [vp, vp, i32, P(ct.c_int64)]),
     "fode_batched_interval_noalloc": (i32, [vp, vp, P(CtrlParams), P(BatchDesc), i32, vp]),

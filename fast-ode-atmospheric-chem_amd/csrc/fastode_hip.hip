@@ -521,6 +521,73 @@ __global__ __launch_bounds__(256) void k_be_update(fode_ctrl_params p, fode_grou
   }
 }
 
+// The same update over three launches (work: 2 doubles per cell), for
+// groups whose cells are many enough that one workgroup's loads bound the
+// kernel above.  k_be_cells: workgroup c updates cell c's iterate and
+// writes its two flags (a correction outside its bound, a value not
+// finite); the group's flags are their maxima, so their order is free.
+__global__ __launch_bounds__(256) void k_be_cells(fode_ctrl_params p, int n, double* __restrict__ Yn1,
+                                                  const double* __restrict__ delta, const double* __restrict__ atol,
+                                                  double rtol, double* __restrict__ flags) {
+  __shared__ double red[2 * (NT / 64)];
+  const int c = blockIdx.x;
+  double miss = 0.0, nonfinite = 0.0;
+  for (int i = threadIdx.x; i < n; i += NT) {
+    const size_t o = (size_t)c * n + i;
+    const double d = delta[o];
+    const double v = np_max0(Yn1[o] + d);
+    Yn1[o] = v;
+    const double bound = np_max(np_max(p.small, atol[i]), rtol * fabs(v));
+    if (!isfinite(d) || !(fabs(d) <= bound)) miss = 1.0;
+    if (!isfinite(v)) nonfinite = 1.0;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    miss = fmax(miss, __shfl_xor(miss, off, 64));
+    nonfinite = fmax(nonfinite, __shfl_xor(nonfinite, off, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[2 * (threadIdx.x >> 6)] = miss;
+    red[2 * (threadIdx.x >> 6) + 1] = nonfinite;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < NT / 64; ++w) {
+      miss = fmax(miss, red[2 * w]);
+      nonfinite = fmax(nonfinite, red[2 * w + 1]);
+    }
+    flags[2 * c] = miss;
+    flags[2 * c + 1] = nonfinite;
+  }
+}
+
+// The group's flags, then be_after (one thread).
+__global__ void k_be_decide(fode_ctrl_params p, fode_group_ctrl* g, int C, const double* __restrict__ flags) {
+  if (threadIdx.x != 0) return;
+  double miss = 0.0, nonfinite = 0.0;
+  for (int c = 0; c < C; ++c) {
+    miss = fmax(miss, flags[2 * c]);
+    nonfinite = fmax(nonfinite, flags[2 * c + 1]);
+  }
+  fode_group_ctrl gc = *g;
+  be_after(p, gc, miss == 0.0, nonfinite == 0.0);
+  *g = gc;
+}
+
+// The iterate committed to Yn or reset to it, then clamped when the call ends.
+__global__ void k_be_apply(long N, const fode_group_ctrl* __restrict__ g, double* __restrict__ Yn1,
+                           double* __restrict__ Yn) {
+  const int keep = g->accepted;
+  const bool finish = g->action == FODE_FINISH;
+  if (keep == FODE_BE_KEEP && !finish) return;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+    double v = Yn1[i];
+    if (keep == FODE_BE_COMMIT) Yn[i] = v;
+    else if (keep == FODE_BE_RESTORE) v = Yn[i];
+    if (finish) v = np_max0(v);
+    Yn1[i] = v;
+  }
+}
+
 // k[c, cols[q]] = (base[c, q] * f) * scale[q]: the photolysis rewrite at an
 // interval start, in the reference's order of products (fastode/ts1.py:205-213).
 __global__ void k_scale_columns(int C, int ldk, double* __restrict__ k, int ncols, const int* __restrict__ cols,
@@ -990,11 +1057,21 @@ int fode_be_residual(int C, int n, const fode_group_ctrl* g, const double* F, co
 }
 
 int fode_be_update(const fode_ctrl_params* p, fode_group_ctrl* g, int C, int n, double* Yn1, double* Yn,
-                   const double* delta, const double* atol, double rtol, void* stream) {
+                   const double* delta, const double* atol, double rtol, double* work, void* stream) {
   if (!p || !g || C <= 0 || n <= 0) return fail("fode_be_update: bad argument");
   if (p->loop != FODE_LOOP_BACKWARD_EULER || p->n_reductions < 0 || p->n_reductions > 8)
     return fail("fode_be_update: parameters are not backward Euler's");
-  hipLaunchKernelGGL(k_be_update, dim3(1), dim3(256), 0, S(stream), *p, g, C, n, Yn1, Yn, delta, atol, rtol);
+  if (!work) {
+    hipLaunchKernelGGL(k_be_update, dim3(1), dim3(256), 0, S(stream), *p, g, C, n, Yn1, Yn, delta, atol, rtol);
+    HIP_OK(hipGetLastError());
+    return 0;
+  }
+  const long N = (long)C * n;
+  hipLaunchKernelGGL(k_be_cells, dim3(C), dim3(256), 0, S(stream), *p, n, Yn1, delta, atol, rtol, work);
+  HIP_OK(hipGetLastError());
+  hipLaunchKernelGGL(k_be_decide, dim3(1), dim3(64), 0, S(stream), *p, g, C, work);
+  HIP_OK(hipGetLastError());
+  hipLaunchKernelGGL(k_be_apply, dim3(grid_for(N)), dim3(256), 0, S(stream), N, g, Yn1, Yn);
   HIP_OK(hipGetLastError());
   return 0;
 }

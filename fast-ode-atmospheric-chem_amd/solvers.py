@@ -375,6 +375,7 @@ class BackwardEuler:
         s = L.stream_handle()
         L.check(L.lib.fode_ctrl_init(ct.byref(params), ctrl.ptr, 1, 0, s), "ctrl_init")
         resid = torch.empty_like(Yn)
+        flags = torch.empty(2 * C, dtype=F64, device=Yn.device)    # each cell's convergence flags
         g = ctrl.read()
         while g["action"] != L.FINISH:
             F = system.rhs(Yn1)
@@ -384,8 +385,8 @@ class BackwardEuler:
                                            L.stream_handle()), "fode_be_residual")
             delta = LU.solve(resid)
             L.check(L.lib.fode_be_update(ct.byref(params), ctrl.ptr, C, n, L.ptr(Yn1), L.ptr(Yn),
-                                         L.ptr(delta), L.ptr(atol_d), float(rtol), L.stream_handle()),
-                    "fode_be_update")
+                                         L.ptr(delta), L.ptr(atol_d), float(rtol), L.ptr(flags),
+                                         L.stream_handle()), "fode_be_update")
             g = ctrl.read()
         counts.add_array(g["counts"])
         return _out(Yn1, as_numpy), L.STATUS[int(g["status"])]

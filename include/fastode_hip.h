@@ -190,11 +190,13 @@ int fode_norm_control(const fode_ctrl_params* p, fode_group_ctrl* g, int C, int 
  * group's convergence test (BackwardEuler.converged, :433-438) and the step
  * decision of :462-486 (the iterate kept, committed to Yn or reset to it),
  * clamping Yn1 when the call ends (:491).  The set-up of a call is
- * fode_ctrl_init with p->loop = FODE_LOOP_BACKWARD_EULER. */
+ * fode_ctrl_init with p->loop = FODE_LOOP_BACKWARD_EULER.  work: 2*C
+ * doubles of device memory (each cell's flags, one workgroup per cell), or
+ * NULL for one workgroup over the whole group; both give the same bits. */
 int fode_be_residual(int C, int n, const fode_group_ctrl* g, const double* F, const double* Yn1,
                      const double* Yn, double* out, void* stream);
 int fode_be_update(const fode_ctrl_params* p, fode_group_ctrl* g, int C, int n, double* Yn1, double* Yn,
-                   const double* delta, const double* atol, double rtol, void* stream);
+                   const double* delta, const double* atol, double rtol, double* work, void* stream);
 
 /* The per-interval photolysis rewrite of run_scenario (fastode/ts1.py:205-213)
  * on device rate constants: k[c*ldk + cols[q]] = (base[c*ncols + q] * f[0])

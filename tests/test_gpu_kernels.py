@@ -237,3 +237,50 @@ def test_norm_control_per_cell_shares_equal_one_workgroup(loop):
         assert np.array_equal(y0, y1), (loop, trial)
         seen.add("finish" if rec["action"] == L.FINISH else ("accept" if rec["accepted"] else "reject"))
     assert seen >= ({"accept", "reject", "finish"} if loop == "algorithm" else {"accept", "reject"}), seen
+
+
+def test_be_update_per_cell_flags_equal_one_workgroup():
+    """fode_be_update with a work array (one workgroup per cell writes its
+    flags, one thread takes the decision, a grid commits, resets and clamps)
+    against the one-workgroup form: the same control block, iterate and step
+    start, bit for bit, for a converging pass (committed; the call ends and
+    clamps), a pass that misses its bound, and one with a value not finite."""
+    import ctypes as ct
+    import fastode_amd._lib as L
+    from fastode_amd import BackwardEuler, Memory
+    from fastode_amd.solvers import GroupControl, ctrl_params
+    rng = np.random.default_rng(5)
+    C, n = 40, 300
+    p = ctrl_params(BackwardEuler(), None, False)
+    atol, rtol = dev(np.full(n, 1e-8)), 1e-3
+    s = L.stream_handle()
+    seen = set()
+    for case in ("converge", "miss", "nonfinite"):
+        Yn = rng.uniform(-1e-3, 1.0, (C, n))
+        Yn1 = Yn + 1e-3 * rng.standard_normal((C, n))
+        delta = {"converge": 1e-30, "miss": 1.0, "nonfinite": 1e-30}[case] * rng.standard_normal((C, n))
+        if case == "nonfinite":
+            delta[7, 11] = np.inf
+        forms = []
+        for use_work in (False, True):
+            ctrl = GroupControl(1.0, Memory())
+            L.check(L.lib.fode_ctrl_init(ct.byref(p), ctrl.ptr, 1, 0, s))
+            work = torch.empty(2 * C, dtype=torch.float64, device="cuda") if use_work else None
+            forms.append((ctrl, dev(Yn), dev(Yn1), work))
+        dd = dev(delta)
+        for step in range(14):                   # passes of one call, both forms side by side
+            outs = []
+            for ctrl, Ynd, Yn1d, work in forms:
+                if ctrl.read()["action"] == L.FINISH:
+                    continue
+                L.check(L.lib.fode_be_update(ct.byref(p), ctrl.ptr, C, n, L.ptr(Yn1d), L.ptr(Ynd), L.ptr(dd),
+                                             L.ptr(atol), rtol, L.ptr(work), s))
+                outs.append((ctrl.dev.cpu().numpy().copy(), Yn1d.cpu().numpy(), Ynd.cpu().numpy()))
+                rec = ctrl.read()
+            if not outs:
+                break
+            (g0, a0, b0), (g1, a1, b1) = outs
+            assert np.array_equal(g0, g1), (case, step)
+            assert np.array_equal(a0, a1) and np.array_equal(b0, b1), (case, step)
+            seen.add((int(rec["accepted"]), int(rec["action"])))
+    assert len(seen) >= 3, seen
